@@ -1,0 +1,259 @@
+"""tinypt_amd -- MI355X (gfx950) backend for the tinypt path-tracing hot path.
+
+Host mirror of the reference crate's render entry (lib.rs:38-43, render.rs:42):
+
+    scene = tinypt_amd.scenes.cornell_box(512, 512)
+    cfg = tinypt_amd.RenderConfig(width=512, height=512, spp=256, max_bounces=12, rr_start=4, seed=2)
+    out = tinypt_amd.render(scene, cfg)          # RenderOutput(acc (h,w,3), acc_w (h,w))
+    img = tinypt_amd.resolve_pixels(out.acc, out.acc_w)
+
+Everything runs through the C ABI of ``lib/libtinypt_gpu.so`` (include/tinypt_gpu.h).  There is
+no CPU fallback: without the built library or a HIP device every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import FP32_WAVEFRONT, FP64_PLAIN
+from .scene import (Camera, DiffuseLight, Dielectric, EnvMap, Ggx, Instance, Lambert, Material, Mesh, Metal, RenderConfig,
+                    Scene, Sphere, Subsurface, Transform, World, color_from_srgb, luminance)
+
+__all__ = ["Camera", "DiffuseLight", "Dielectric", "EnvMap", "Ggx", "Instance", "Lambert", "Material", "Mesh", "Metal",
+           "RenderConfig", "Scene", "Sphere", "Subsurface", "Transform", "World", "Context", "DeviceScene", "HostScene",
+           "RenderOutput",
+           "render", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
+
+import atexit
+import weakref
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libtinypt_gpu.so")
+_lib = None
+
+
+class TptError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tinypt_gpu error {code}: {msg}")
+        self.code = code
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def load_library():
+    """Load the built C-ABI library (never builds; run tinypt_amd.build first)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise TptError(_abi.TPT_ERR_INTERNAL, f"{_LIB_PATH} is not built (python -m tinypt_amd.build)")
+        L = C.CDLL(_LIB_PATH)
+        for name, (res, args) in _abi.ENTRY_POINTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        if L.tpt_abi_version() != 1:
+            raise TptError(_abi.TPT_ERR_INTERNAL, "ABI version mismatch")
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise TptError(rc, load_library().tpt_last_error().decode(errors="replace"))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# Live handles, released in dependency order before the HIP runtime tears itself down at exit.
+_live_scenes: "weakref.WeakSet" = weakref.WeakSet()
+_live_ctxs: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    for s in list(_live_scenes):
+        s.close()
+    for c in list(_live_ctxs):
+        c.close()
+
+
+class Context:
+    """A HIP device context (one stream).  ``tpt_ctx_create``."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        _check(self._lib.tpt_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+        _live_ctxs.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.tpt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class _SceneTables:
+    """Queries shared by host-only and uploaded scenes."""
+
+    def info(self) -> dict:
+        inf = _abi.SceneInfo()
+        _check(self._lib.tpt_scene_get_info(self.h, C.byref(inf)))
+        return inf.as_dict()
+
+    def bvh(self, mesh: int) -> dict:
+        """The mesh's BVH as built (bvh.rs:24-43 layout)."""
+        n = C.c_int64()
+        _check(self._lib.tpt_scene_get_bvh(self.h, mesh, C.byref(n), None, None, None, None, None, None))
+        n = n.value
+        ntri = len(self.scene.world.meshes[mesh].tris)
+        bbox = np.zeros((max(n, 1), 6))
+        left = np.zeros(max(n, 1), np.int32)
+        right = np.zeros(max(n, 1), np.int32)
+        start = np.zeros(max(n, 1), np.uint32)
+        count = np.zeros(max(n, 1), np.uint32)
+        idx = np.zeros(max(ntri, 1), np.int64)
+        nn = C.c_int64()
+        _check(self._lib.tpt_scene_get_bvh(self.h, mesh, C.byref(nn), _dp(bbox), left.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           right.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           start.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           count.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           idx.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(bbox=bbox[:n], left=left[:n], right=right[:n], start=start[:n], count=count[:n], indices=idx[:ntri])
+
+    def lights(self) -> np.ndarray:
+        """(n, 10): kind, sphere, mesh, tri, instance, emit rgb, weight, cdf (world.rs:127-167)."""
+        n = C.c_int32()
+        _check(self._lib.tpt_scene_get_lights(self.h, C.byref(n), None))
+        out = np.zeros((max(n.value, 1), 10))
+        _check(self._lib.tpt_scene_get_lights(self.h, C.byref(n), _dp(out)))
+        return out[:n.value]
+
+    def env_tables(self) -> dict:
+        """Alias table and pdf numerators of an image environment."""
+        env = self.scene.env
+        N = env.width * env.height
+        total = C.c_double()
+        prob = np.zeros(N, np.float32)
+        alias = np.zeros(N, np.int32)
+        pdf = np.zeros(N, np.float32)
+        _check(self._lib.tpt_scene_get_env_tables(self.h, C.byref(total), prob.ctypes.data_as(C.POINTER(C.c_float)),
+                                                  alias.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                  pdf.ctypes.data_as(C.POINTER(C.c_float))))
+        return dict(total=total.value, prob=prob, alias=alias, cell_pdf=pdf)
+
+
+class HostScene(_SceneTables):
+    """Host-side scene tables only (``tpt_scene_build_host``): no device needed, cannot render."""
+
+    def __init__(self, scene: Scene):
+        self._lib = load_library()
+        self.scene = scene
+        desc, keep = scene.to_desc()
+        h = C.c_void_p()
+        _check(self._lib.tpt_scene_build_host(C.byref(desc), C.byref(h)))
+        self.h = h
+        _live_scenes.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.tpt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceScene(_SceneTables):
+    """A scene uploaded to the device (``tpt_scene_upload``): BVHs built as Bvh::build does,
+    top-level tree, lights, environment tables."""
+
+    def __init__(self, scene: Scene, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self._lib = self.ctx._lib
+        self.scene = scene
+        desc, keep = scene.to_desc()
+        h = C.c_void_p()
+        _check(self._lib.tpt_scene_upload(self.ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h
+        self.n_materials = len(scene.mats)
+        _live_scenes.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.tpt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def update_materials(self, mat_param: np.ndarray):
+        """Replace every material's 4 parameters (``tpt_scene_update_materials``)."""
+        p = np.ascontiguousarray(np.asarray(mat_param, dtype=np.float64).reshape(self.n_materials, 4))
+        _check(self._lib.tpt_scene_update_materials(self.ctx.h, self.h, _dp(p)))
+
+    def trace_closest(self, rays: np.ndarray) -> dict:
+        """FP32 closest hit (World::hit semantics).  rays (n, 8): o, d, tmin, tmax."""
+        r = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(-1, 8))
+        n = len(r)
+        t = np.zeros(n, np.float32)
+        kind = np.zeros(n, np.int32)
+        inst = np.zeros(n, np.int32)
+        prim = np.zeros(n, np.int32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        _check(self._lib.tpt_trace_closest(self.ctx.h, self.h, n, r.ctypes.data_as(C.POINTER(C.c_float)),
+                                           t.ctypes.data_as(C.POINTER(C.c_float)), ip(kind), ip(inst), ip(prim)))
+        return dict(t=t, kind=kind, inst=inst, prim=prim)
+
+
+@dataclass
+class RenderOutput:
+    """render.rs:34-39 plus per-pixel sums of squares and run counters."""
+    acc: np.ndarray     # (h, w, 3) sum of path radiance
+    acc_w: np.ndarray   # (h, w) samples per pixel
+    acc_sq: Optional[np.ndarray]
+    stats: dict
+
+
+def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Context] = None) -> RenderOutput:
+    """render() (render.rs:42-278) on the GPU.  ``scene`` is a Scene or an uploaded
+    DeviceScene.  Checkpointing is the caller's business (the buffers are returned)."""
+    dscene = scene if isinstance(scene, DeviceScene) else DeviceScene(scene, ctx)
+    lib = dscene._lib
+    h, w = int(config.height), int(config.width)
+    acc = np.zeros((h, w, 3))
+    acc_w = np.zeros((h, w))
+    acc_sq = np.zeros((h, w, 3)) if want_sq else None
+    st = _abi.Stats()
+    cfg = config.to_c()
+    _check(lib.tpt_render(dscene.ctx.h, dscene.h, C.byref(cfg), _dp(acc), _dp(acc_w),
+                          _dp(acc_sq) if want_sq else None, C.byref(st)))
+    return RenderOutput(acc, acc_w, acc_sq, st.as_dict())
+
+
+def resolve_pixels(acc: np.ndarray, acc_w: np.ndarray) -> np.ndarray:
+    """output.rs:96-102: acc / max(acc_w, 1)."""
+    return acc / np.maximum(acc_w, 1.0)[..., None]

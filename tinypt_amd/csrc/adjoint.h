@@ -1,0 +1,18 @@
+// adjoint.h -- path-replay adjoint of the FP32 wavefront render (k_adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/tinypt_gpu.h"
+#include "tpt_kernels.h"
+#include "tpt_trace32.h"
+
+namespace tpt {
+
+int adjoint_render(hipStream_t s, const SceneF32& S, const EnvDev& env, const LightsDev& lights, const MatsDev& mats,
+                   const CamF32& cam, int gstack_cap, const tpt_render_config* cfg, const float* d_loss_d_pixel,
+                   double* grad_params, tpt_stats* stats, std::string* err);
+
+}  // namespace tpt

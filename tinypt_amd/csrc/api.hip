@@ -1,0 +1,727 @@
+// api.hip -- the C ABI (include/tinypt_gpu.h): contexts, scene upload, render dispatch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tinypt_gpu.h"
+#include "adjoint.h"
+#include "plain64.h"
+#include "scene_host.h"
+#include "wavefront.h"
+
+using namespace tpt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(x)                                                                                     \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) return fail(TPT_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t n) {  // grow-only
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, n ? n : 16);
+    if (e == hipSuccess) bytes = n ? n : 16;
+    return e;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  hipError_t e = b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+}
+
+}  // namespace
+
+struct tpt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // wavefront scratch (grow-only)
+  DevBuf ps_o, ps_d, ps_beta, ps_L, ps_rng, ps_pix, ps_hit, sh_o, sh_d, sh_c, q0, q1, sq, counters, stats, gstack;
+  // adaptive / plain per-pixel state
+  DevBuf act0, act1, nsamp, welford, rngpix, done;
+  size_t ps_cap = 0;
+};
+
+struct tpt_scene {
+  tpt_ctx* ctx = nullptr;  // null for a host-only scene (tpt_scene_build_host)
+  HostScene host;
+  DevBuf nodes2, tris32, tri_mat32, tri_orig, inst32, tlas_items, sph64, sph_mat;
+  DevBuf ref_nodes, ref_indices, tri_v, tri_mat, meshes, inst64;
+  DevBuf mat_kind, mat_p32, mat_p64;
+  DevBuf l32, cdf32, l64, cdf64;
+  DevBuf env_rgb32, alias_prob, alias_idx, cell_pdf, env_rgb64, row_cdf, col_cdf;
+  SceneF32 s32{};
+  SceneF64 s64{};
+  EnvDev env{};
+  LightsDev lights{};
+  MatsDev mats{};
+  CamF32 cam32{};
+  size_t device_bytes = 0;
+};
+
+static int upload_materials(tpt_scene* sc) {
+  hipStream_t s = sc->ctx->stream;
+  const HostScene& H = sc->host;
+  std::vector<float4> p32(H.mat_kind.size());
+  for (size_t i = 0; i < p32.size(); i++)
+    p32[i] = make_float4((float)H.mat_param[4 * i], (float)H.mat_param[4 * i + 1], (float)H.mat_param[4 * i + 2],
+                         (float)H.mat_param[4 * i + 3]);
+  HIPC(upload(sc->mat_kind, H.mat_kind, s));
+  HIPC(upload(sc->mat_p32, p32, s));
+  HIPC(upload(sc->mat_p64, H.mat_param, s));
+  sc->mats.n = (int32_t)H.mat_kind.size();
+  sc->mats.kind = sc->mat_kind.as<int32_t>();
+  sc->mats.p32 = sc->mat_p32.as<float4>();
+  sc->mats.p64 = sc->mat_p64.as<double>();
+  return TPT_OK;
+}
+
+static int upload_lights(tpt_scene* sc) {
+  hipStream_t s = sc->ctx->stream;
+  const HostScene& H = sc->host;
+  HIPC(upload(sc->l32, H.lights32, s));
+  HIPC(upload(sc->cdf32, H.light_cdf32, s));
+  HIPC(upload(sc->l64, H.lights, s));
+  HIPC(upload(sc->cdf64, H.light_cdf, s));
+  sc->lights.n = (int32_t)H.lights.size();
+  sc->lights.total = H.light_total;
+  sc->lights.total32 = (float)H.light_total;
+  sc->lights.l32 = sc->l32.as<LightF32>();
+  sc->lights.cdf32 = sc->cdf32.as<float>();
+  sc->lights.l64 = sc->l64.as<LightF64>();
+  sc->lights.cdf64 = sc->cdf64.as<double>();
+  return TPT_OK;
+}
+
+extern "C" {
+
+int tpt_abi_version(void) { return TPT_ABI_VERSION; }
+
+const char* tpt_last_error(void) { return g_err.c_str(); }
+
+int tpt_ctx_create(int device, tpt_ctx** out) {
+  if (!out) return fail(TPT_ERR_INVALID_ARGUMENT, "out is null");
+  *out = nullptr;
+  int n = 0;
+  HIPC(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(TPT_ERR_INVALID_ARGUMENT, "no such HIP device");
+  HIPC(hipSetDevice(device));
+  std::unique_ptr<tpt_ctx> c(new tpt_ctx());
+  c->device = device;
+  HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return TPT_OK;
+}
+
+int tpt_ctx_destroy(tpt_ctx* ctx) {
+  if (!ctx) return TPT_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TPT_OK;
+}
+
+int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) {
+  if (!ctx || !desc || !out) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  HIPC(hipSetDevice(ctx->device));
+  std::unique_ptr<tpt_scene> sc(new tpt_scene());
+  sc->ctx = ctx;
+  HostScene& H = sc->host;
+  try {
+    if (!H.build(desc)) return fail(TPT_ERR_INVALID_ARGUMENT, H.error);
+  } catch (const std::bad_alloc&) {
+    return fail(TPT_ERR_OUT_OF_MEMORY, "host allocation failed while building the scene");
+  }
+  hipStream_t s = ctx->stream;
+  std::vector<double> sph64 = H.sph;
+  HIPC(upload(sc->nodes2, H.nodes2, s));
+  HIPC(upload(sc->tris32, H.tris32, s));
+  HIPC(upload(sc->tri_mat32, H.tri_mat32, s));
+  HIPC(upload(sc->tri_orig, H.tri_orig, s));
+  HIPC(upload(sc->inst32, H.inst32, s));
+  HIPC(upload(sc->tlas_items, H.tlas_items, s));
+  HIPC(upload(sc->sph64, sph64, s));
+  HIPC(upload(sc->sph_mat, H.sph_mat, s));
+  HIPC(upload(sc->ref_nodes, H.ref_nodes, s));
+  HIPC(upload(sc->ref_indices, H.ref_indices, s));
+  HIPC(upload(sc->tri_v, H.tri_v, s));
+  HIPC(upload(sc->tri_mat, H.tri_mat, s));
+  HIPC(upload(sc->meshes, H.meshes, s));
+  HIPC(upload(sc->inst64, H.inst64, s));
+  int rc = upload_materials(sc.get());
+  if (rc) return rc;
+  rc = upload_lights(sc.get());
+  if (rc) return rc;
+  // environment
+  EnvDev& E = sc->env;
+  E.kind = H.env_kind;
+  E.w = H.env_w;
+  E.h = H.env_h;
+  E.total = H.env_total;
+  if (H.env_kind == TPT_ENV_IMAGE) {
+    std::vector<float4> rgb(H.env_rgb32.size() / 4);
+    std::memcpy(rgb.data(), H.env_rgb32.data(), sizeof(float) * H.env_rgb32.size());
+    HIPC(upload(sc->env_rgb32, rgb, s));
+    HIPC(upload(sc->alias_prob, H.alias_prob, s));
+    HIPC(upload(sc->alias_idx, H.alias_idx, s));
+    HIPC(upload(sc->cell_pdf, H.cell_pdf, s));
+    HIPC(upload(sc->env_rgb64, H.env_rgb, s));
+    HIPC(upload(sc->row_cdf, H.env_row_cdf, s));
+    HIPC(upload(sc->col_cdf, H.env_col_cdf, s));
+    E.rgb32 = sc->env_rgb32.as<float4>();
+    E.alias_prob = sc->alias_prob.as<float>();
+    E.alias_idx = sc->alias_idx.as<int32_t>();
+    E.cell_pdf = sc->cell_pdf.as<float>();
+    E.rgb64 = sc->env_rgb64.as<double>();
+    E.row_cdf = sc->row_cdf.as<double>();
+    E.col_cdf = sc->col_cdf.as<double>();
+  }
+  // FP32 view
+  SceneF32& S = sc->s32;
+  S.nodes = sc->nodes2.as<Node2>();
+  S.tris = sc->tris32.as<TriF32>();
+  S.tri_mat = sc->tri_mat32.as<int32_t>();
+  S.tri_orig = sc->tri_orig.as<int32_t>();
+  S.inst = sc->inst32.as<InstF32>();
+  S.tlas_items = sc->tlas_items.as<int32_t>();
+  S.sph = sc->sph64.as<double>();
+  S.sph_mat = sc->sph_mat.as<int32_t>();
+  S.tlas_root = H.tlas_root;
+  S.n_spheres = (int32_t)H.sph_mat.size();
+  // FP64 view
+  SceneF64& D = sc->s64;
+  D.nodes = sc->ref_nodes.as<RefNodeDev>();
+  D.indices = sc->ref_indices.as<int32_t>();
+  D.tris = sc->tri_v.as<double>();
+  D.tri_mat = sc->tri_mat.as<int32_t>();
+  D.meshes = sc->meshes.as<MeshDev>();
+  D.inst = sc->inst64.as<InstF64>();
+  D.sph = sc->sph64.as<double>();
+  D.sph_mat = sc->sph_mat.as<int32_t>();
+  D.n_inst = (int32_t)H.inst64.size();
+  D.n_meshes = (int32_t)H.meshes.size();
+  D.n_spheres = (int32_t)H.sph_mat.size();
+  // camera
+  CamF32& C = sc->cam32;
+  for (int k = 0; k < 3; k++) {
+    C.o[k] = (float)H.cam.o[k];
+    C.u[k] = (float)H.cam.u[k];
+    C.v[k] = (float)H.cam.v[k];
+    C.w[k] = (float)H.cam.w[k];
+  }
+  C.half_h = (float)H.cam.half_h;
+  C.half_w = (float)H.cam.half_w;
+  C.lens_radius = (float)H.cam.lens_radius;
+  C.focus_dist = (float)H.cam.focus_dist;
+  HIPC(hipStreamSynchronize(s));
+  for (DevBuf* b : {&sc->nodes2, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->inst32, &sc->tlas_items, &sc->sph64,
+                    &sc->sph_mat, &sc->ref_nodes, &sc->ref_indices, &sc->tri_v, &sc->tri_mat, &sc->meshes, &sc->inst64,
+                    &sc->mat_kind, &sc->mat_p32, &sc->mat_p64, &sc->l32, &sc->cdf32, &sc->l64, &sc->cdf64, &sc->env_rgb32,
+                    &sc->alias_prob, &sc->alias_idx, &sc->cell_pdf, &sc->env_rgb64, &sc->row_cdf, &sc->col_cdf})
+    sc->device_bytes += b->bytes;
+  *out = sc.release();
+  return TPT_OK;
+}
+
+int tpt_scene_destroy(tpt_scene* scene) {
+  if (!scene) return TPT_OK;
+  if (scene->ctx) {
+    hipSetDevice(scene->ctx->device);
+    hipStreamSynchronize(scene->ctx->stream);
+  }
+  delete scene;
+  return TPT_OK;
+}
+
+int tpt_scene_build_host(const tpt_scene_desc* desc, tpt_scene** out) {
+  if (!desc || !out) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::unique_ptr<tpt_scene> sc(new tpt_scene());
+  try {
+    if (!sc->host.build(desc)) return fail(TPT_ERR_INVALID_ARGUMENT, sc->host.error);
+  } catch (const std::bad_alloc&) {
+    return fail(TPT_ERR_OUT_OF_MEMORY, "host allocation failed while building the scene");
+  }
+  *out = sc.release();
+  return TPT_OK;
+}
+
+int tpt_scene_get_lights(const tpt_scene* sc, int32_t* n_lights, double* out) {
+  if (!sc || !n_lights) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  const HostScene& H = sc->host;
+  *n_lights = (int32_t)H.lights.size();
+  if (!out) return TPT_OK;
+  for (size_t i = 0; i < H.lights.size(); i++) {
+    const LightF64& L = H.lights[i];
+    double* o = out + 10 * i;
+    o[0] = L.kind; o[1] = L.idx; o[2] = L.mesh; o[3] = L.tri; o[4] = L.inst;
+    o[5] = L.emit[0]; o[6] = L.emit[1]; o[7] = L.emit[2]; o[8] = L.weight; o[9] = H.light_cdf[i];
+  }
+  return TPT_OK;
+}
+
+int tpt_scene_get_env_tables(const tpt_scene* sc, double* total_weight, float* alias_prob, int32_t* alias_idx,
+                             float* cell_pdf) {
+  if (!sc) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  const HostScene& H = sc->host;
+  if (H.env_kind != TPT_ENV_IMAGE) return fail(TPT_ERR_INVALID_ARGUMENT, "scene has no image environment");
+  if (total_weight) *total_weight = H.env_total;
+  size_t n = H.alias_prob.size();
+  if (alias_prob) std::memcpy(alias_prob, H.alias_prob.data(), n * sizeof(float));
+  if (alias_idx) std::memcpy(alias_idx, H.alias_idx.data(), n * sizeof(int32_t));
+  if (cell_pdf) std::memcpy(cell_pdf, H.cell_pdf.data(), n * sizeof(float));
+  return TPT_OK;
+}
+
+int tpt_scene_get_info(const tpt_scene* sc, tpt_scene_info* info) {
+  if (!sc || !info) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  const HostScene& H = sc->host;
+  info->n_tris = (int64_t)H.tri_mat.size();
+  int64_t blas_nodes = 0;
+  for (const RefBvh& b : H.blas) blas_nodes += (int64_t)b.nodes.size();
+  info->n_blas_nodes = blas_nodes;
+  info->n_tlas_nodes = (int64_t)H.tlas_items.size();
+  info->max_blas_depth = H.max_blas_depth;
+  info->max_tlas_depth = H.max_tlas_depth;
+  info->n_lights = (int32_t)H.lights.size();
+  info->reserved = 0;
+  info->device_bytes = (int64_t)sc->device_bytes;
+  return TPT_OK;
+}
+
+int tpt_scene_get_bvh(const tpt_scene* sc, int32_t mesh, int64_t* n_nodes, double* node_bbox, int32_t* node_left,
+                      int32_t* node_right, uint32_t* node_start, uint32_t* node_count, int64_t* indices) {
+  if (!sc || !n_nodes) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (mesh < 0 || mesh >= (int32_t)sc->host.blas.size()) return fail(TPT_ERR_INVALID_ARGUMENT, "mesh out of range");
+  const RefBvh& b = sc->host.blas[(size_t)mesh];
+  *n_nodes = (int64_t)b.nodes.size();
+  if (!node_bbox) return TPT_OK;
+  for (size_t i = 0; i < b.nodes.size(); i++) {
+    const RefNode& n = b.nodes[i];
+    for (int k = 0; k < 3; k++) {
+      node_bbox[6 * i + (size_t)k] = n.bbox.mn[k];
+      node_bbox[6 * i + 3 + (size_t)k] = n.bbox.mx[k];
+    }
+    node_left[i] = n.left;
+    node_right[i] = n.right;
+    node_start[i] = n.start;
+    node_count[i] = n.count;
+  }
+  if (indices)
+    for (size_t i = 0; i < b.indices.size(); i++) indices[i] = b.indices[i];
+  return TPT_OK;
+}
+
+int tpt_scene_update_materials(tpt_ctx* ctx, tpt_scene* sc, const double* mat_param) {
+  if (!ctx || !sc || !mat_param) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (sc->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  HIPC(hipSetDevice(ctx->device));
+  HostScene& H = sc->host;
+  std::memcpy(H.mat_param.data(), mat_param, sizeof(double) * H.mat_param.size());
+  H.rebuild_lights();
+  int rc = upload_materials(sc);
+  if (rc) return rc;
+  rc = upload_lights(sc);
+  if (rc) return rc;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return TPT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- render
+namespace {
+
+int check_cfg(const tpt_render_config* c) {
+  if (!c) return fail(TPT_ERR_INVALID_ARGUMENT, "config is null");
+  if (c->width <= 0 || c->height <= 0) return fail(TPT_ERR_INVALID_ARGUMENT, "width and height must be positive");
+  if ((int64_t)c->width * c->height > (int64_t)1 << 28) return fail(TPT_ERR_INVALID_ARGUMENT, "image too large");
+  if (c->spp < 0 || c->max_bounces < 0 || c->max_bounces > 255 || c->rr_start < 0)
+    return fail(TPT_ERR_INVALID_ARGUMENT, "spp, max_bounces (<= 255) and rr_start must be non-negative");
+  if (c->precision != TPT_FP32_WAVEFRONT && c->precision != TPT_FP64_PLAIN)
+    return fail(TPT_ERR_INVALID_ARGUMENT, "unknown precision");
+  return TPT_OK;
+}
+
+int needed_gstack(const HostScene& H) {
+  int need = H.max_tlas_depth + H.max_blas_depth + 2;
+  return need > TPT_WF_STACK ? need - TPT_WF_STACK : 0;
+}
+
+int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
+  if (P > c->ps_cap) {
+    HIPC(c->ps_o.ensure(P * 16));
+    HIPC(c->ps_d.ensure(P * 16));
+    HIPC(c->ps_beta.ensure(P * 16));
+    HIPC(c->ps_L.ensure(P * 16));
+    HIPC(c->ps_rng.ensure(P * 8));
+    HIPC(c->ps_pix.ensure(P * 4));
+    HIPC(c->ps_hit.ensure(P * 16));
+    HIPC(c->sh_o.ensure(P * 32));
+    HIPC(c->sh_d.ensure(P * 32));
+    HIPC(c->sh_c.ensure(P * 32));
+    HIPC(c->q0.ensure(P * 4));
+    HIPC(c->q1.ensure(P * 4));
+    HIPC(c->sq.ensure(P * 4));
+    c->ps_cap = P;
+  }
+  HIPC(c->counters.ensure(TPT_NUM_CNT * 4));
+  HIPC(c->stats.ensure(8 * 8));
+  if (gcap > 0) HIPC(c->gstack.ensure((size_t)wf_grid_threads((uint32_t)P) * (size_t)gcap * 4));
+  return TPT_OK;
+}
+
+WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, int gcap) {
+  WfParams W{};
+  W.S = sc->s32;
+  W.env = sc->env;
+  W.lights = sc->lights;
+  W.mats = sc->mats;
+  W.cam = sc->cam32;
+  PathState& ps = W.ps;
+  ps.o = c->ps_o.as<float4>();
+  ps.d = c->ps_d.as<float4>();
+  ps.beta = c->ps_beta.as<float4>();
+  ps.L = c->ps_L.as<float4>();
+  ps.rng = c->ps_rng.as<uint64_t>();
+  ps.pix = c->ps_pix.as<uint32_t>();
+  ps.hit = c->ps_hit.as<int4>();
+  ps.sh_o = c->sh_o.as<float4>();
+  ps.sh_d = c->sh_d.as<float4>();
+  ps.sh_c = c->sh_c.as<float4>();
+  ps.q[0] = c->q0.as<int32_t>();
+  ps.q[1] = c->q1.as<int32_t>();
+  ps.sq = c->sq.as<int32_t>();
+  ps.counters = c->counters.as<uint32_t>();
+  ps.stats = c->stats.as<uint64_t>();
+  W.gstack = gcap > 0 ? c->gstack.as<int32_t>() : nullptr;
+  W.gstack_cap = gcap;
+  W.width = cfg->width;
+  W.height = cfg->height;
+  W.max_bounces = cfg->max_bounces;
+  W.rr_start = cfg->rr_start;
+  W.inv_w = (float)(1.0 / (double)cfg->width);
+  W.inv_h = (float)(1.0 / (double)cfg->height);
+  W.seed = cfg->seed;
+  return W;
+}
+
+struct EventTimer {
+  hipStream_t s;
+  bool on;
+  std::vector<hipEvent_t> ev;
+  EventTimer(hipStream_t st, bool enable) : s(st), on(enable) {}
+  ~EventTimer() {
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+  }
+  hipError_t mark() {
+    if (!on) return hipSuccess;
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    ev.push_back(e);
+    return hipEventRecord(e, s);
+  }
+  double pairs_ms() {  // sum of (ev[2i+1] - ev[2i])
+    double t = 0;
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) t += ms;
+    }
+    return t;
+  }
+};
+
+// Bounce loop of one pass (P paths generated by the preceding raygen).
+int wf_bounces(tpt_ctx* c, const WfParams& W, uint32_t P, EventTimer& tt, uint64_t* launches) {
+  hipStream_t s = c->stream;
+  int cur = 0;
+  for (int b = 0; b < W.max_bounces; b++) {
+    HIPC(tt.mark());
+    HIPC(wf_launch_extend(W, cur, P, s));
+    HIPC(tt.mark());
+    (*launches)++;
+    HIPC(wf_launch_shade(W, cur, P, s));
+    HIPC(wf_launch_shadow(W, P, s));
+    cur ^= 1;
+  }
+  return TPT_OK;
+}
+
+int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, double* acc, double* acc_w,
+                     double* acc_sq, tpt_stats* st) {
+  hipStream_t s = c->stream;
+  const uint32_t npix = (uint32_t)cfg->width * (uint32_t)cfg->height;
+  const uint32_t spp = (uint32_t)cfg->spp;
+  const uint32_t target = 1u << 22;  // paths in flight
+  uint32_t k = cfg->samples_per_pass > 0 ? (uint32_t)cfg->samples_per_pass : std::max(1u, target / std::max(npix, 1u));
+  k = std::max(1u, std::min(k, std::max(spp, 1u)));
+  uint32_t pix_chunk = npix;
+  if ((uint64_t)npix * k > (uint64_t)1 << 25) {  // very large images: split the pixels
+    k = 1;
+    pix_chunk = std::min(npix, 1u << 25);
+  }
+  const int gcap = needed_gstack(sc->host);
+  size_t P = (size_t)pix_chunk * k;
+  if (cfg->adaptive_enabled) P = std::max(P, (size_t)std::min<uint64_t>((uint64_t)npix * 64, (uint64_t)1 << 23));
+  int rc = ensure_paths(c, P, gcap);
+  if (rc) return rc;
+  WfParams W = make_wf(c, sc, cfg, gcap);
+  HIPC(hipMemsetAsync(W.ps.stats, 0, 8 * 8, s));
+  EventTimer total(s, st != nullptr), tt(s, st != nullptr);
+  uint64_t launches = 0, traced = 0;
+  HIPC(total.mark());
+  if (!cfg->adaptive_enabled) {
+    for (uint32_t p0 = 0; p0 < npix; p0 += pix_chunk) {
+      uint32_t np = std::min(pix_chunk, npix - p0);
+      for (uint32_t s0 = 0; s0 < spp; s0 += k) {
+        uint32_t kk = std::min(k, spp - s0);
+        RaygenArgs A{np * kk, np, p0, s0, nullptr, nullptr};
+        HIPC(wf_launch_raygen(W, A, s));
+        rc = wf_bounces(c, W, np * kk, tt, &launches);
+        if (rc) return rc;
+        traced += (uint64_t)np * kk;
+        AccumArgs B{np, p0, kk, acc, acc_w, acc_sq};
+        HIPC(wf_launch_accumulate(W, B, s));
+      }
+    }
+  } else {
+    // render.rs:143-179 per pixel; each pass gives every active pixel k more samples
+    HIPC(c->act0.ensure((size_t)npix * 4));
+    HIPC(c->act1.ensure((size_t)npix * 4));
+    HIPC(c->nsamp.ensure((size_t)npix * 4));
+    HIPC(c->welford.ensure((size_t)npix * 16));
+    HIPC(hipMemsetAsync(c->nsamp.p, 0, (size_t)npix * 4, s));
+    HIPC(hipMemsetAsync(c->welford.p, 0, (size_t)npix * 16, s));
+    std::vector<int32_t> all(npix);
+    for (uint32_t i = 0; i < npix; i++) all[i] = (int32_t)i;
+    HIPC(hipMemcpyAsync(c->act0.p, all.data(), (size_t)npix * 4, hipMemcpyHostToDevice, s));
+    uint32_t n_active = spp > 0 ? npix : 0;
+    uint32_t taken = 0;
+    const uint32_t max_spp = std::max(spp, 1u);
+    const uint32_t min_spp = std::min((uint32_t)std::max(cfg->adaptive_min_spp, 1), max_spp);
+    int32_t* act_in = c->act0.as<int32_t>();
+    int32_t* act_out = c->act1.as<int32_t>();
+    while (n_active > 0 && taken < spp) {
+      uint32_t kk = taken < min_spp ? min_spp - taken : 1;
+      uint32_t fill = (uint32_t)std::max<size_t>(1, P / n_active);
+      kk = std::max(kk, std::min<uint32_t>(fill, 64));
+      kk = std::min(kk, spp - taken);
+      kk = std::min<uint32_t>(kk, (uint32_t)std::max<size_t>(1, P / n_active));
+      RaygenArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>()};
+      HIPC(wf_launch_raygen(W, A, s));
+      rc = wf_bounces(c, W, n_active * kk, tt, &launches);
+      if (rc) return rc;
+      traced += (uint64_t)n_active * kk;
+      uint32_t* n_out = W.ps.counters + TPT_CNT_ACTIVE;
+      HIPC(hipMemsetAsync(n_out, 0, 4, s));
+      AdaptiveArgs B{n_active, kk, min_spp, spp, cfg->adaptive_threshold, act_in, act_out, n_out,
+                     c->nsamp.as<uint32_t>(), c->welford.as<double>(), acc, acc_w, acc_sq};
+      HIPC(wf_launch_accumulate_adaptive(W, B, s));
+      HIPC(hipMemcpyAsync(&n_active, n_out, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      taken += kk;
+      std::swap(act_in, act_out);
+    }
+  }
+  HIPC(total.mark());
+  HIPC(hipStreamSynchronize(s));
+  if (st) {
+    uint64_t h[8] = {0};
+    HIPC(hipMemcpy(h, W.ps.stats, 8 * 8, hipMemcpyDeviceToHost));
+    st->paths = h[2];
+    st->rays_extension = h[0];
+    st->rays_shadow = h[1];
+    st->elapsed_ms = total.pairs_ms();
+    st->trace_kernel_ms = tt.pairs_ms();
+    st->trace_launches = launches;
+    st->trace_rays = h[0];
+  }
+  (void)traced;
+  return TPT_OK;
+}
+
+int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, double* acc, double* acc_w, double* acc_sq,
+                 tpt_stats* st) {
+  if (sc->host.max_blas_depth > 63)
+    return fail(TPT_ERR_UNSUPPORTED, "the plain FP64 form keeps the reference's 64-entry BVH stack; tree too deep");
+  hipStream_t s = c->stream;
+  const size_t npix = (size_t)cfg->width * cfg->height;
+  HIPC(c->rngpix.ensure(npix * 8));
+  HIPC(c->nsamp.ensure(npix * 4));
+  HIPC(c->welford.ensure(npix * 16));
+  HIPC(c->done.ensure(npix));
+  HIPC(c->stats.ensure(64));
+  HIPC(hipMemsetAsync(c->done.p, 0, npix, s));
+  HIPC(hipMemsetAsync(c->stats.p, 0, 64, s));
+  Plain64Params P{};
+  P.S = sc->s64;
+  P.env = sc->env;
+  P.lights = sc->lights;
+  P.mats = sc->mats;
+  P.cam = sc->host.cam;
+  P.width = cfg->width;
+  P.height = cfg->height;
+  P.spp = cfg->spp;
+  P.max_bounces = cfg->max_bounces;
+  P.rr_start = cfg->rr_start;
+  P.adaptive = cfg->adaptive_enabled;
+  P.min_spp = cfg->adaptive_min_spp;
+  P.threshold = cfg->adaptive_threshold;
+  P.inv_w = 1.0 / (double)cfg->width;
+  P.inv_h = 1.0 / (double)cfg->height;
+  P.seed = cfg->seed;
+  P.rng = c->rngpix.as<uint64_t>();
+  P.acc = acc;
+  P.acc_sq = acc_sq;
+  P.acc_w = acc_w;
+  P.nsamp = c->nsamp.as<uint32_t>();
+  P.welford = c->welford.as<double>();
+  P.done = c->done.as<uint8_t>();
+  P.stats = c->stats.as<uint64_t>();
+  EventTimer total(s, st != nullptr);
+  HIPC(total.mark());
+  const int chunk = 8;
+  if (cfg->spp == 0) {
+    P.s_begin = 0;
+    P.s_end = 0;
+    HIPC(launch_plain64(P, s));
+  }
+  for (int s0 = 0; s0 < cfg->spp; s0 += chunk) {
+    P.s_begin = s0;
+    P.s_end = std::min(cfg->spp, s0 + chunk);
+    HIPC(launch_plain64(P, s));
+  }
+  HIPC(total.mark());
+  HIPC(hipStreamSynchronize(s));
+  if (st) {
+    uint64_t h[8] = {0};
+    HIPC(hipMemcpy(h, c->stats.p, 64, hipMemcpyDeviceToHost));
+    st->paths = h[2];
+    st->rays_extension = h[0];
+    st->rays_shadow = h[1];
+    st->elapsed_ms = total.pairs_ms();
+    st->trace_kernel_ms = st->elapsed_ms;
+    st->trace_launches = (uint64_t)((cfg->spp + chunk - 1) / chunk);
+    st->trace_rays = h[0] + h[1];
+  }
+  return TPT_OK;
+}
+
+int render_device(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, double* acc, double* acc_w, double* acc_sq,
+                  tpt_stats* st) {
+  const size_t npix = (size_t)cfg->width * cfg->height;
+  hipStream_t s = c->stream;
+  HIPC(hipMemsetAsync(acc, 0, npix * 24, s));
+  HIPC(hipMemsetAsync(acc_w, 0, npix * 8, s));
+  if (acc_sq) HIPC(hipMemsetAsync(acc_sq, 0, npix * 24, s));
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (cfg->precision == TPT_FP64_PLAIN) return render_plain(c, sc, cfg, acc, acc_w, acc_sq, st);
+  return render_wavefront(c, sc, cfg, acc, acc_w, acc_sq, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tpt_render_device(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, double* d_acc_rgb, double* d_acc_w,
+                      double* d_acc_sq, tpt_stats* stats) {
+  if (!ctx || !scene || !d_acc_rgb || !d_acc_w) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (scene->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  HIPC(hipSetDevice(ctx->device));
+  return render_device(ctx, scene, cfg, d_acc_rgb, d_acc_w, d_acc_sq, stats);
+}
+
+int tpt_render(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, double* acc_rgb, double* acc_w,
+               double* acc_sq, tpt_stats* stats) {
+  if (!ctx || !scene || !acc_rgb || !acc_w) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (scene->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  HIPC(hipSetDevice(ctx->device));
+  const size_t npix = (size_t)cfg->width * cfg->height;
+  DevBuf a, w, q;
+  HIPC(a.ensure(npix * 24));
+  HIPC(w.ensure(npix * 8));
+  if (acc_sq) HIPC(q.ensure(npix * 24));
+  rc = render_device(ctx, scene, cfg, a.as<double>(), w.as<double>(), acc_sq ? q.as<double>() : nullptr, stats);
+  if (rc) return rc;
+  HIPC(hipMemcpy(acc_rgb, a.p, npix * 24, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(acc_w, w.p, npix * 8, hipMemcpyDeviceToHost));
+  if (acc_sq) HIPC(hipMemcpy(acc_sq, q.p, npix * 24, hipMemcpyDeviceToHost));
+  return TPT_OK;
+}
+
+int tpt_trace_closest(tpt_ctx* ctx, const tpt_scene* sc, int64_t n, const float* rays, float* hit_t, int32_t* hit_kind,
+                      int32_t* hit_inst, int32_t* hit_prim) {
+  if (!ctx || !sc || n < 0 || (n > 0 && (!rays || !hit_t || !hit_kind || !hit_inst || !hit_prim)))
+    return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (sc->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  if (n == 0) return TPT_OK;
+  HIPC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  DevBuf r, t, k, i, p, g;
+  HIPC(r.ensure((size_t)n * 32));
+  HIPC(t.ensure((size_t)n * 4));
+  HIPC(k.ensure((size_t)n * 4));
+  HIPC(i.ensure((size_t)n * 4));
+  HIPC(p.ensure((size_t)n * 4));
+  int gcap = needed_gstack(sc->host);
+  if (gcap > 0) HIPC(g.ensure((size_t)trace_grid_threads(n) * (size_t)gcap * 4));
+  HIPC(hipMemcpyAsync(r.p, rays, (size_t)n * 32, hipMemcpyHostToDevice, s));
+  HIPC(launch_trace_closest(sc->s32, n, r.as<float>(), t.as<float>(), k.as<int32_t>(), i.as<int32_t>(), p.as<int32_t>(),
+                            gcap > 0 ? g.as<int32_t>() : nullptr, gcap, s));
+  HIPC(hipMemcpyAsync(hit_t, t.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(hit_kind, k.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(hit_inst, i.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(hit_prim, p.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  return TPT_OK;
+}
+
+int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, const float* d_loss_d_pixel,
+                       double* grad_params, tpt_stats* stats) {
+  if (!ctx || !scene || !d_loss_d_pixel || !grad_params) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (scene->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  HIPC(hipSetDevice(ctx->device));
+  return adjoint_render(ctx->stream, scene->s32, scene->env, scene->lights, scene->mats, scene->cam32,
+                        needed_gstack(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
+}
+
+}  // extern "C"

@@ -1,0 +1,626 @@
+// scene_host.cpp -- host-side scene build (see scene_host.h).
+#include "scene_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <numeric>
+
+namespace tpt {
+
+static const double kPi = 3.14159265358979323846;
+static const double kInf = std::numeric_limits<double>::infinity();
+
+// ------------------------------------------------------------------ f64 helpers
+static inline double fmx(double a, double b) { return std::fmax(a, b); }
+static inline double fmn(double a, double b) { return std::fmin(a, b); }
+static inline size_t sat_usize(double v) {  // Rust `as usize`
+  if (!(v > 0.0)) return 0;
+  if (v >= 18446744073709551615.0) return SIZE_MAX;
+  return (size_t)v;
+}
+
+static DAabb empty_box() {
+  DAabb b;
+  for (int k = 0; k < 3; k++) { b.mn[k] = kInf; b.mx[k] = -kInf; }
+  return b;
+}
+static DAabb unite(const DAabb& a, const DAabb& b) {
+  DAabb r;
+  for (int k = 0; k < 3; k++) { r.mn[k] = fmn(a.mn[k], b.mn[k]); r.mx[k] = fmx(a.mx[k], b.mx[k]); }
+  return r;
+}
+static DAabb grow(const DAabb& a, const double* p) {
+  DAabb r;
+  for (int k = 0; k < 3; k++) { r.mn[k] = fmn(a.mn[k], p[k]); r.mx[k] = fmx(a.mx[k], p[k]); }
+  return r;
+}
+
+// Aabb::from_points (geometry.rs:70-80) and Triangle::bounds (geometry.rs:221-225): for a
+// static triangle the shutter-open and shutter-close boxes are equal, so their union is one.
+DAabb triangle_bounds(const double* v) {
+  DAabb b = grow(grow(grow(empty_box(), v), v + 3), v + 6);
+  const double eps = 1e-9;
+  for (int k = 0; k < 3; k++) { b.mn[k] = b.mn[k] - eps; b.mx[k] = b.mx[k] + eps; }
+  return b;
+}
+
+// ------------------------------------------------------------------ reference BVH build
+// The steps of Bvh::build (bvh.rs:51-242) over primitive boxes: pre-order node array, leaf if
+// n <= leaf_size, split axis = largest centroid extent, 8-bin SAH when n >= 16 with a stable
+// sort of the range by bin, otherwise the range is cut at n/2 in its current order.  The
+// arithmetic is the reference's f64 arithmetic so the node array comes out identical.
+namespace {
+struct RefBuilder {
+  const std::vector<DAabb>& bounds;
+  std::vector<double> cent;  // 3 per primitive
+  size_t leaf_size;
+  RefBvh out;
+  static const size_t BINS = 8;
+
+  RefBuilder(const std::vector<DAabb>& b, size_t ls) : bounds(b), leaf_size(ls) {
+    cent.resize(3 * b.size());
+    for (size_t i = 0; i < b.size(); i++)
+      for (int k = 0; k < 3; k++) cent[3 * i + k] = (b[i].mn[k] + b[i].mx[k]) * 0.5;  // Aabb::centroid
+  }
+
+  int32_t node(size_t start, size_t end, int depth) {
+    out.depth = std::max(out.depth, depth);
+    size_t n = end - start;
+    DAabb bbox = empty_box(), cbox = empty_box();
+    for (size_t k = start; k < end; k++) {
+      size_t i = (size_t)out.indices[k];
+      bbox = unite(bbox, bounds[i]);
+      cbox = grow(cbox, &cent[3 * i]);
+    }
+    int32_t idx = (int32_t)out.nodes.size();
+    out.nodes.push_back(RefNode{bbox, -1, -1, 0, 0});
+    if (n <= leaf_size) {
+      out.nodes[(size_t)idx].start = (uint32_t)start;
+      out.nodes[(size_t)idx].count = (uint32_t)n;
+      return idx;
+    }
+    double ex = cbox.mx[0] - cbox.mn[0], ey = cbox.mx[1] - cbox.mn[1], ez = cbox.mx[2] - cbox.mn[2];
+    int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+    size_t mid = start + n / 2;
+    if (n >= BINS * 2) {
+      double minc = cbox.mn[axis], maxc = cbox.mx[axis];
+      double extent = maxc - minc;
+      if (extent > 1e-12) {
+        double inv_extent = 1.0 / extent;
+        auto bin_of = [&](int64_t prim) {
+          size_t b = sat_usize((cent[3 * (size_t)prim + axis] - minc) * inv_extent * (double)BINS);
+          return b >= BINS ? BINS - 1 : b;
+        };
+        DAabb bb[BINS];
+        size_t bc[BINS];
+        for (size_t b = 0; b < BINS; b++) { bb[b] = empty_box(); bc[b] = 0; }
+        for (size_t k = start; k < end; k++) {
+          size_t b = bin_of(out.indices[k]);
+          bb[b] = unite(bb[b], bounds[(size_t)out.indices[k]]);
+          bc[b] += 1;
+        }
+        DAabb lb[BINS], rb[BINS];
+        size_t lc[BINS], rc[BINS];
+        DAabb acc = empty_box();
+        size_t an = 0;
+        for (size_t b = 0; b < BINS; b++) {
+          if (bc[b] > 0) { acc = unite(acc, bb[b]); an += bc[b]; }
+          lb[b] = acc;
+          lc[b] = an;
+        }
+        acc = empty_box();
+        an = 0;
+        for (size_t b = BINS; b-- > 0;) {
+          if (bc[b] > 0) { acc = unite(acc, bb[b]); an += bc[b]; }
+          rb[b] = acc;
+          rc[b] = an;
+        }
+        double best_cost = kInf;
+        size_t best_split = 0;
+        for (size_t b = 0; b + 1 < BINS; b++) {
+          if (lc[b] == 0 || rc[b + 1] == 0) continue;
+          double lx = lb[b].mx[0] - lb[b].mn[0], ly = lb[b].mx[1] - lb[b].mn[1], lz = lb[b].mx[2] - lb[b].mn[2];
+          double rx = rb[b + 1].mx[0] - rb[b + 1].mn[0], ry = rb[b + 1].mx[1] - rb[b + 1].mn[1],
+                 rz = rb[b + 1].mx[2] - rb[b + 1].mn[2];
+          double la = 2.0 * (lx * ly + ly * lz + lz * lx);
+          double ra = 2.0 * (rx * ry + ry * rz + rz * rx);
+          double cost = la * (double)lc[b] + ra * (double)rc[b + 1];
+          if (cost < best_cost) { best_cost = cost; best_split = b + 1; }
+        }
+        if (std::isfinite(best_cost)) {
+          std::stable_sort(out.indices.begin() + (std::ptrdiff_t)start, out.indices.begin() + (std::ptrdiff_t)end,
+                           [&](int64_t a, int64_t b) { return bin_of(a) < bin_of(b); });
+          size_t left_total = 0;
+          for (size_t b = 0; b < best_split; b++) left_total += bc[b];
+          if (left_total > 0 && left_total < n) mid = start + left_total;
+        }
+      }
+    }
+    int32_t l = node(start, mid, depth + 1);
+    int32_t r = node(mid, end, depth + 1);
+    out.nodes[(size_t)idx].left = l;
+    out.nodes[(size_t)idx].right = r;
+    return idx;
+  }
+};
+}  // namespace
+
+RefBvh build_ref_bvh(const std::vector<DAabb>& prim_bounds, size_t leaf_size) {
+  RefBuilder b(prim_bounds, leaf_size);
+  b.out.indices.resize(prim_bounds.size());
+  std::iota(b.out.indices.begin(), b.out.indices.end(), 0);
+  if (!prim_bounds.empty()) b.node(0, prim_bounds.size(), 1);
+  return std::move(b.out);
+}
+
+// ------------------------------------------------------------------ FP32 conversion
+static inline float round_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+  return f;
+}
+static inline float round_up(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+  return f;
+}
+static void put_box(float* b, const DAabb& x) {
+  for (int k = 0; k < 3; k++) { b[k] = round_down(x.mn[k]); b[3 + k] = round_up(x.mx[k]); }
+}
+
+// Reference BVH -> Node2 array.  Returns the child ref of `id`.  Leaf slots are offset by
+// `slot_base` (position of the mesh's first reordered primitive).
+static int32_t to_node2(const RefBvh& r, int32_t id, int64_t slot_base, std::vector<Node2>& out, std::string& err) {
+  const RefNode& n = r.nodes[(size_t)id];
+  if (n.left == -1 && n.right == -1) {
+    int64_t s = slot_base + (int64_t)n.start;
+    if (n.count == 0 || n.count > 8 || s > (int64_t)TPT_LEAF_START_MASK) {
+      err = "leaf does not fit the node encoding";
+      return TPT_REF_EMPTY;
+    }
+    return (int32_t)(TPT_LEAF_BIT | ((uint32_t)(n.count - 1) << 28) | (uint32_t)s);
+  }
+  int32_t me = (int32_t)out.size();
+  out.push_back(Node2{});
+  Node2 tmp{};
+  put_box(tmp.b, r.nodes[(size_t)n.left].bbox);
+  put_box(tmp.b + 6, r.nodes[(size_t)n.right].bbox);
+  tmp.child[0] = to_node2(r, n.left, slot_base, out, err);
+  tmp.child[1] = to_node2(r, n.right, slot_base, out, err);
+  out[(size_t)me] = tmp;
+  return me;
+}
+
+// ------------------------------------------------------------------ TLAS (own design)
+// Full-sweep SAH over item boxes, one item per leaf.  The reference has no top-level tree; it
+// visits every instance and then every sphere for each ray (world.rs:86-118).
+namespace {
+struct TlasBuilder {
+  const std::vector<DAabb>& box;
+  std::vector<int32_t> order;
+  std::vector<Node2>& nodes;
+  int depth = 0;
+  TlasBuilder(const std::vector<DAabb>& b, std::vector<Node2>& n) : box(b), nodes(n) {}
+  static double area(const DAabb& b) {
+    double x = b.mx[0] - b.mn[0], y = b.mx[1] - b.mn[1], z = b.mx[2] - b.mn[2];
+    return 2.0 * (x * y + y * z + z * x);
+  }
+  DAabb range_box(size_t lo, size_t hi) const {
+    DAabb b = empty_box();
+    for (size_t i = lo; i < hi; i++) b = unite(b, box[(size_t)order[i]]);
+    return b;
+  }
+  int32_t build(size_t lo, size_t hi, int d) {
+    depth = std::max(depth, d);
+    size_t n = hi - lo;
+    if (n == 1) return (int32_t)(TPT_LEAF_BIT | (uint32_t)lo);
+    double best = kInf;
+    int best_axis = 0;
+    size_t best_mid = lo + n / 2;
+    std::vector<double> right_area(n + 1);
+    for (int axis = 0; axis < 3; axis++) {
+      std::stable_sort(order.begin() + (std::ptrdiff_t)lo, order.begin() + (std::ptrdiff_t)hi, [&](int32_t a, int32_t b) {
+        return box[(size_t)a].mn[axis] + box[(size_t)a].mx[axis] < box[(size_t)b].mn[axis] + box[(size_t)b].mx[axis];
+      });
+      DAabb acc = empty_box();
+      for (size_t k = n; k-- > 1;) {
+        acc = unite(acc, box[(size_t)order[lo + k]]);
+        right_area[k] = area(acc);
+      }
+      acc = empty_box();
+      for (size_t k = 1; k < n; k++) {
+        acc = unite(acc, box[(size_t)order[lo + k - 1]]);
+        double cost = area(acc) * (double)k + right_area[k] * (double)(n - k);
+        if (cost < best) { best = cost; best_axis = axis; best_mid = lo + k; }
+      }
+    }
+    std::stable_sort(order.begin() + (std::ptrdiff_t)lo, order.begin() + (std::ptrdiff_t)hi, [&](int32_t a, int32_t b) {
+      return box[(size_t)a].mn[best_axis] + box[(size_t)a].mx[best_axis] <
+             box[(size_t)b].mn[best_axis] + box[(size_t)b].mx[best_axis];
+    });
+    int32_t me = (int32_t)nodes.size();
+    nodes.push_back(Node2{});
+    Node2 tmp{};
+    put_box(tmp.b, range_box(lo, best_mid));
+    put_box(tmp.b + 6, range_box(best_mid, hi));
+    tmp.child[0] = build(lo, best_mid, d + 1);
+    tmp.child[1] = build(best_mid, hi, d + 1);
+    nodes[(size_t)me] = tmp;
+    return me;
+  }
+};
+}  // namespace
+
+// ------------------------------------------------------------------ matrices (math.rs:100-170)
+static void mat_invert(const double* m, double* r) {
+  double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+  double inv_det = 1.0 / det;
+  r[0] = (m[4] * m[8] - m[5] * m[7]) * inv_det;
+  r[1] = (m[2] * m[7] - m[1] * m[8]) * inv_det;
+  r[2] = (m[1] * m[5] - m[2] * m[4]) * inv_det;
+  r[3] = (m[5] * m[6] - m[3] * m[8]) * inv_det;
+  r[4] = (m[0] * m[8] - m[2] * m[6]) * inv_det;
+  r[5] = (m[2] * m[3] - m[0] * m[5]) * inv_det;
+  r[6] = (m[3] * m[7] - m[4] * m[6]) * inv_det;
+  r[7] = (m[1] * m[6] - m[0] * m[7]) * inv_det;
+  r[8] = (m[0] * m[4] - m[1] * m[3]) * inv_det;
+}
+static void mat_vec(const double* m, const double* v, double* out) {
+  out[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  out[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+  out[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+static double lum(const double* c) { return 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]; }
+
+// ------------------------------------------------------------------ build
+bool HostScene::build(const tpt_scene_desc* d) {
+  const int nm = d->n_materials, ns = d->n_spheres, nmesh = d->n_meshes, ni = d->n_instances;
+  if (nm < 0 || ns < 0 || nmesh < 0 || ni < 0) { error = "negative counts in scene descriptor"; return false; }
+  if ((nm && (!d->mat_kind || !d->mat_param)) || (ns && (!d->sphere_center || !d->sphere_radius || !d->sphere_mat)) ||
+      (nmesh && !d->mesh_tri_begin) || (ni && (!d->inst_mesh || !d->inst_xform || !d->inst_mat_override))) {
+    error = "null array in scene descriptor";
+    return false;
+  }
+  // ---- camera: Camera::look_at_dof (ray.rs:59-86)
+  {
+    const tpt_camera_desc& c = d->camera;
+    double theta = c.vfov_deg * (kPi / 180.0);
+    cam.half_h = std::tan(theta * 0.5);
+    cam.half_w = c.aspect * cam.half_h;
+    double w[3] = {c.eye[0] - c.target[0], c.eye[1] - c.target[1], c.eye[2] - c.target[2]};
+    double wl = fmx(std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]), 1e-30);
+    for (int k = 0; k < 3; k++) cam.w[k] = w[k] / wl;
+    double u[3] = {c.up[1] * cam.w[2] - c.up[2] * cam.w[1], c.up[2] * cam.w[0] - c.up[0] * cam.w[2],
+                   c.up[0] * cam.w[1] - c.up[1] * cam.w[0]};
+    double ul = fmx(std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), 1e-30);
+    for (int k = 0; k < 3; k++) cam.u[k] = u[k] / ul;
+    cam.v[0] = cam.w[1] * cam.u[2] - cam.w[2] * cam.u[1];
+    cam.v[1] = cam.w[2] * cam.u[0] - cam.w[0] * cam.u[2];
+    cam.v[2] = cam.w[0] * cam.u[1] - cam.w[1] * cam.u[0];
+    for (int k = 0; k < 3; k++) cam.o[k] = c.eye[k];
+    cam.lens_radius = 0.5 * fmx(c.aperture, 0.0);
+    cam.focus_dist = fmx(c.focus_dist, 1e-6);
+    cam.shutter_open = 0.0;
+    cam.shutter_close = 1.0;
+  }
+  // ---- copies + validation
+  mat_kind.assign(d->mat_kind, d->mat_kind + nm);
+  mat_param.assign(d->mat_param, d->mat_param + 4 * (size_t)nm);
+  for (int i = 0; i < nm; i++)
+    if (mat_kind[(size_t)i] < TPT_MAT_LAMBERT || mat_kind[(size_t)i] > TPT_MAT_DIFFUSE_LIGHT) {
+      error = "unknown material kind";
+      return false;
+    }
+  sph.resize(4 * (size_t)ns);
+  sph_mat.assign(d->sphere_mat, d->sphere_mat + ns);
+  for (int i = 0; i < ns; i++) {
+    for (int k = 0; k < 3; k++) sph[4 * (size_t)i + k] = d->sphere_center[3 * i + k];
+    sph[4 * (size_t)i + 3] = d->sphere_radius[i];
+    if (sph_mat[(size_t)i] < 0 || sph_mat[(size_t)i] >= nm) { error = "sphere material out of range"; return false; }
+  }
+  mesh_tri_begin.assign(d->mesh_tri_begin, d->mesh_tri_begin + (nmesh ? nmesh + 1 : 0));
+  int64_t ntri = nmesh ? mesh_tri_begin[(size_t)nmesh] : 0;
+  if (nmesh && mesh_tri_begin[0] != 0) { error = "mesh_tri_begin[0] must be 0"; return false; }
+  for (int m = 0; m < nmesh; m++)
+    if (mesh_tri_begin[(size_t)m + 1] < mesh_tri_begin[(size_t)m]) { error = "mesh_tri_begin not monotone"; return false; }
+  if (ntri > 0 && (!d->tri_vertices || !d->tri_mat)) { error = "null triangle arrays"; return false; }
+  if (ntri > (int64_t)TPT_LEAF_START_MASK) { error = "too many triangles"; return false; }
+  tri_v.assign(d->tri_vertices, d->tri_vertices + 9 * (size_t)ntri);
+  tri_mat.assign(d->tri_mat, d->tri_mat + (size_t)ntri);
+  for (int64_t t = 0; t < ntri; t++)
+    if (tri_mat[(size_t)t] < 0 || tri_mat[(size_t)t] >= nm) { error = "triangle material out of range"; return false; }
+  inst_mesh.assign(d->inst_mesh, d->inst_mesh + ni);
+  inst_xform.assign(d->inst_xform, d->inst_xform + 12 * (size_t)ni);
+  inst_override.assign(d->inst_mat_override, d->inst_mat_override + ni);
+  for (int i = 0; i < ni; i++)
+    if (inst_override[(size_t)i] < -1 || inst_override[(size_t)i] >= nm) { error = "instance material override out of range"; return false; }
+  env_kind = d->env_kind;
+  if (env_kind == TPT_ENV_IMAGE) {
+    env_w = d->env_width;
+    env_h = d->env_height;
+    if (env_w <= 0 || env_h <= 0 || !d->env_rgb) { error = "image environment needs width, height and pixels"; return false; }
+    env_rgb.assign(d->env_rgb, d->env_rgb + 3 * (size_t)env_w * (size_t)env_h);
+  } else if (env_kind != TPT_ENV_SKY) {
+    error = "unknown env_kind";
+    return false;
+  }
+
+  // ---- per-mesh BLAS: reference build, reference layout, reordered FP32 triangles, Node2
+  blas.resize((size_t)nmesh);
+  meshes.resize((size_t)nmesh);
+  mesh_root2.assign((size_t)nmesh, TPT_REF_EMPTY);
+  tris32.resize((size_t)ntri);
+  tri_orig.resize((size_t)ntri);
+  tri_mat32.resize((size_t)ntri);
+  for (int m = 0; m < nmesh; m++) {
+    int64_t b0 = mesh_tri_begin[(size_t)m], b1 = mesh_tri_begin[(size_t)m + 1];
+    std::vector<DAabb> bounds((size_t)(b1 - b0));
+    for (int64_t t = b0; t < b1; t++) bounds[(size_t)(t - b0)] = triangle_bounds(&tri_v[9 * (size_t)t]);
+    blas[(size_t)m] = build_ref_bvh(bounds, 4);  // LEAF_SIZE (constants.rs:21)
+    const RefBvh& r = blas[(size_t)m];
+    max_blas_depth = std::max(max_blas_depth, r.depth);
+    MeshDev md{};
+    md.node_off = (int32_t)ref_nodes.size();
+    md.idx_off = (int32_t)ref_indices.size();
+    md.tri_off = (int32_t)b0;
+    md.n_nodes = (int32_t)r.nodes.size();
+    md.n_tris = (int32_t)(b1 - b0);
+    for (const RefNode& n : r.nodes) {
+      RefNodeDev x{};
+      for (int k = 0; k < 3; k++) { x.bbox[k] = n.bbox.mn[k]; x.bbox[3 + k] = n.bbox.mx[k]; }
+      x.left = n.left; x.right = n.right; x.start = n.start; x.count = n.count;
+      ref_nodes.push_back(x);
+    }
+    for (int64_t ix : r.indices) ref_indices.push_back((int32_t)ix);
+    // reordered FP32 triangles: slot b0 + k holds triangle indices[k]
+    for (size_t k = 0; k < r.indices.size(); k++) {
+      int64_t t = b0 + r.indices[k];
+      const double* v = &tri_v[9 * (size_t)t];
+      TriF32& o = tris32[(size_t)(b0 + (int64_t)k)];
+      for (int c = 0; c < 3; c++) {
+        o.v0[c] = (float)v[c];
+        o.e1[c] = (float)(v[3 + c] - v[c]);
+        o.e2[c] = (float)(v[6 + c] - v[c]);
+      }
+      o.v0[3] = o.e1[3] = o.e2[3] = 0.f;
+      tri_orig[(size_t)(b0 + (int64_t)k)] = (int32_t)r.indices[k];
+      tri_mat32[(size_t)(b0 + (int64_t)k)] = tri_mat[(size_t)t];
+    }
+    if (!r.nodes.empty()) {
+      mesh_root2[(size_t)m] = to_node2(r, 0, b0, nodes2, error);
+      if (!error.empty()) return false;
+    }
+    meshes[(size_t)m] = md;
+  }
+  for (int m = 0; m < nmesh; m++) meshes[(size_t)m].pad = mesh_root2[(size_t)m];
+
+  // ---- instances
+  inst64.resize((size_t)ni);
+  inst32.resize((size_t)ni);
+  std::vector<DAabb> item_box;
+  std::vector<int32_t> item_code;
+  for (int i = 0; i < ni; i++) {
+    InstF64& x = inst64[(size_t)i];
+    const double* src = &inst_xform[12 * (size_t)i];
+    for (int k = 0; k < 9; k++) x.a[k] = src[k];
+    for (int k = 0; k < 3; k++) x.t[k] = src[9 + k];
+    mat_invert(x.a, x.a_inv);  // Transform::from_affine (transform.rs:24-27)
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) x.nrm[3 * r + c] = x.a_inv[3 * c + r];
+    x.mesh = inst_mesh[(size_t)i];
+    x.mat_override = inst_override[(size_t)i];
+    InstF32& y = inst32[(size_t)i];
+    double ait[3];
+    mat_vec(x.a_inv, x.t, ait);
+    for (int r = 0; r < 3; r++) {
+      for (int c = 0; c < 3; c++) {
+        y.w2o[4 * r + c] = (float)x.a_inv[3 * r + c];
+        y.nrm[4 * r + c] = (float)x.nrm[3 * r + c];
+      }
+      y.w2o[4 * r + 3] = (float)(-ait[r]);
+      y.nrm[4 * r + 3] = 0.f;
+    }
+    bool valid = x.mesh >= 0 && x.mesh < nmesh;  // world.rs:87-90 skips unknown meshes
+    y.root = valid ? mesh_root2[(size_t)x.mesh] : TPT_REF_EMPTY;
+    y.mat_override = x.mat_override;
+    y.mesh = valid ? x.mesh : -1;
+    y.pad = 0;
+    if (!valid || blas[(size_t)x.mesh].nodes.empty()) continue;
+    // world box: the eight corners of the mesh root box through A, t; padded
+    const DAabb& ob = blas[(size_t)x.mesh].nodes[0].bbox;
+    DAabb wb = empty_box();
+    for (int c = 0; c < 8; c++) {
+      double p[3] = {(c & 1) ? ob.mx[0] : ob.mn[0], (c & 2) ? ob.mx[1] : ob.mn[1], (c & 4) ? ob.mx[2] : ob.mn[2]};
+      double q[3];
+      mat_vec(x.a, p, q);
+      for (int k = 0; k < 3; k++) q[k] += x.t[k];
+      wb = grow(wb, q);
+    }
+    double ext = std::max({wb.mx[0] - wb.mn[0], wb.mx[1] - wb.mn[1], wb.mx[2] - wb.mn[2], 1e-6});
+    for (int k = 0; k < 3; k++) {
+      double pad = 1e-6 * ext + 1e-6 * std::max(std::fabs(wb.mn[k]), std::fabs(wb.mx[k]));
+      wb.mn[k] -= pad;
+      wb.mx[k] += pad;
+    }
+    item_box.push_back(wb);
+    item_code.push_back(i);
+  }
+  sph32.resize(4 * (size_t)ns);
+  for (int s = 0; s < ns; s++) {
+    const double* c = &sph[4 * (size_t)s];
+    double r = std::fabs(c[3]);
+    DAabb b;
+    for (int k = 0; k < 3; k++) {
+      double pad = 1e-6 * (r + std::fabs(c[k]));
+      b.mn[k] = c[k] - r - pad;
+      b.mx[k] = c[k] + r + pad;
+      sph32[4 * (size_t)s + k] = (float)c[k];
+    }
+    sph32[4 * (size_t)s + 3] = (float)c[3];
+    item_box.push_back(b);
+    item_code.push_back((int32_t)(TPT_ITEM_SPHERE | (uint32_t)s));
+  }
+  // ---- TLAS
+  if (!item_box.empty()) {
+    TlasBuilder tb(item_box, nodes2);
+    tb.order.resize(item_box.size());
+    std::iota(tb.order.begin(), tb.order.end(), 0);
+    tlas_root = tb.build(0, item_box.size(), 1);
+    max_tlas_depth = tb.depth;
+    tlas_items.resize(item_box.size());
+    for (size_t k = 0; k < tb.order.size(); k++) tlas_items[k] = item_code[(size_t)tb.order[k]];
+  }
+  if (nodes2.size() > (size_t)INT32_MAX / 2) { error = "too many nodes"; return false; }
+
+  // ---- environment: EnvMap::from_pixels (env.rs:61-89) + alias table over the same weights
+  if (env_kind == TPT_ENV_IMAGE) {
+    size_t w = (size_t)env_w, h = (size_t)env_h;
+    env_row_cdf.assign(h + 1, 0.0);
+    env_col_cdf.assign(h * (w + 1), 0.0);
+    std::vector<double> cell(w * h);
+    double total = 0.0;
+    for (size_t y = 0; y < h; y++) {
+      double theta0 = kPi * (double)y / (double)h;
+      double theta1 = kPi * (double)(y + 1) / (double)h;
+      double row_weight = fmx(std::cos(theta0) - std::cos(theta1), 0.0);
+      size_t off = y * (w + 1);
+      double row_sum = 0.0;
+      for (size_t x = 0; x < w; x++) {
+        double l = fmx(lum(&env_rgb[3 * (y * w + x)]), 0.0);
+        cell[y * w + x] = l * row_weight;
+        row_sum += l * row_weight;
+        env_col_cdf[off + x + 1] = row_sum;
+      }
+      total += row_sum;
+      env_row_cdf[y + 1] = total;
+    }
+    env_total = total;
+    env_rgb32.resize(4 * w * h);
+    for (size_t i = 0; i < w * h; i++) {
+      for (int k = 0; k < 3; k++) env_rgb32[4 * i + (size_t)k] = (float)env_rgb[3 * i + (size_t)k];
+      env_rgb32[4 * i + 3] = 0.f;
+    }
+    size_t N = w * h;
+    alias_prob.assign(N, 1.0f);
+    alias_idx.resize(N);
+    cell_pdf.assign(N, 0.0f);
+    for (size_t i = 0; i < N; i++) alias_idx[i] = (int32_t)i;
+    if (total > 0.0) {
+      // Vose's alias method (Vose 1991) in FP64.
+      std::vector<double> q(N);
+      std::vector<size_t> small, large;
+      for (size_t i = 0; i < N; i++) {
+        q[i] = cell[i] / total * (double)N;
+        cell_pdf[i] = (float)((cell[i] / total) * (double)w * (double)h / (2.0 * kPi * kPi));
+        (q[i] < 1.0 ? small : large).push_back(i);
+      }
+      while (!small.empty() && !large.empty()) {
+        size_t s = small.back(); small.pop_back();
+        size_t l = large.back(); large.pop_back();
+        alias_prob[s] = (float)q[s];
+        alias_idx[s] = (int32_t)l;
+        q[l] = (q[l] + q[s]) - 1.0;
+        (q[l] < 1.0 ? small : large).push_back(l);
+      }
+      for (size_t i : large) { alias_prob[i] = 1.0f; alias_idx[i] = (int32_t)i; }
+      for (size_t i : small) {  // leftovers from rounding: keep only a positive-weight cell
+        alias_prob[i] = 1.0f;
+        alias_idx[i] = (int32_t)i;
+      }
+    }
+  }
+  rebuild_lights();
+  return error.empty();
+}
+
+// World::build_lights (world.rs:127-167): spheres first, then every instance's triangles whose
+// effective material emits; weight = area(t=0.5) * luminance(emit); cumulative CDF.
+void HostScene::rebuild_lights() {
+  lights.clear();
+  light_cdf.clear();
+  lights32.clear();
+  light_cdf32.clear();
+  double total = 0.0;
+  auto emits = [&](int mid) { return mid >= 0 && mid < (int)mat_kind.size() && mat_kind[(size_t)mid] == TPT_MAT_DIFFUSE_LIGHT; };
+  auto add = [&](LightF64 L, double area, const double* world_v) {
+    const double* e = &mat_param[4 * (size_t)(L.kind == 0 ? sph_mat[(size_t)L.idx]
+                                                          : (inst_override[(size_t)L.inst] >= 0 ? inst_override[(size_t)L.inst]
+                                                                                                : tri_mat[(size_t)(mesh_tri_begin[(size_t)L.mesh] + L.tri)]))];
+    double weight = area * lum(e);
+    if (!(weight > 0.0)) return;
+    total += weight;
+    for (int k = 0; k < 3; k++) L.emit[k] = e[k];
+    L.weight = weight;
+    lights.push_back(L);
+    light_cdf.push_back(total);
+    LightF32 f{};
+    f.kind = L.kind;
+    if (L.kind == 0) {
+      for (int k = 0; k < 4; k++) f.p0[k] = (float)sph[4 * (size_t)L.idx + (size_t)k];
+    } else {
+      for (int k = 0; k < 3; k++) {
+        f.p0[k] = (float)world_v[k];
+        f.p1[k] = (float)world_v[3 + k];
+        f.p2[k] = (float)world_v[6 + k];
+      }
+    }
+    for (int k = 0; k < 3; k++) f.emit[k] = (float)e[k];
+    f.emit[3] = (float)weight;
+    lights32.push_back(f);
+  };
+  for (size_t s = 0; s < sph_mat.size(); s++) {
+    if (!emits(sph_mat[s])) continue;
+    double r = sph[4 * s + 3];
+    LightF64 L{};
+    L.kind = 0;
+    L.idx = (int32_t)s;
+    add(L, 4.0 * kPi * r * r, nullptr);
+  }
+  // per mesh, the triangles whose own material emits (instances without an override)
+  std::vector<std::vector<int32_t>> emissive(meshes.size());
+  for (size_t m = 0; m < meshes.size(); m++)
+    for (int64_t t = mesh_tri_begin[m]; t < mesh_tri_begin[m + 1]; t++)
+      if (emits(tri_mat[(size_t)t])) emissive[m].push_back((int32_t)(t - mesh_tri_begin[m]));
+  for (size_t i = 0; i < inst64.size(); i++) {
+    const InstF64& x = inst64[i];
+    if (x.mesh < 0 || x.mesh >= (int)meshes.size()) continue;
+    std::vector<int32_t> all;
+    const std::vector<int32_t>* list = &emissive[(size_t)x.mesh];
+    if (x.mat_override >= 0) {
+      if (!emits(x.mat_override)) continue;
+      all.resize((size_t)meshes[(size_t)x.mesh].n_tris);
+      std::iota(all.begin(), all.end(), 0);
+      list = &all;
+    }
+    for (int32_t t : *list) {
+      // tri_world_verts at time 0.5 (world.rs:276-292): vertices_at(0.5) of a static triangle
+      // is v*0.5 + v*0.5 = v exactly, then apply_point.
+      const double* v = &tri_v[9 * (size_t)(mesh_tri_begin[(size_t)x.mesh] + t)];
+      double wv[9];
+      for (int j = 0; j < 3; j++) {
+        double vt[3];
+        for (int k = 0; k < 3; k++) vt[k] = v[3 * j + k] * (1.0 - 0.5) + v[3 * j + k] * 0.5;
+        mat_vec(x.a, vt, wv + 3 * j);
+        for (int k = 0; k < 3; k++) wv[3 * j + k] = wv[3 * j + k] + x.t[k];
+      }
+      double e1[3] = {wv[3] - wv[0], wv[4] - wv[1], wv[5] - wv[2]};
+      double e2[3] = {wv[6] - wv[0], wv[7] - wv[1], wv[8] - wv[2]};
+      double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+      double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+      LightF64 L{};
+      L.kind = 1;
+      L.mesh = x.mesh;
+      L.tri = t;
+      L.inst = (int32_t)i;
+      add(L, area, wv);
+    }
+  }
+  light_total = total;
+  for (double c : light_cdf) light_cdf32.push_back((float)c);
+}
+
+}  // namespace tpt

@@ -1,0 +1,212 @@
+// tpt_trace32.h -- FP32 closest-hit / any-hit traversal of the two-level scene (device).
+//
+// Semantics follow World::hit (world.rs:81-121) with Bvh::hit (bvh.rs:248-351), Triangle::hit
+// (geometry.rs:236-273) and Sphere::hit (geometry.rs:174-195):
+//   * instance rays are taken to object space (A^-1 (o - t), A^-1 d) but d is NOT normalised,
+//     so object-space t equals world t and the running closest distance culls every level;
+//     the reference's object-space bound t_obj >= tmin on the normalised ray becomes
+//     t >= tmin / |A^-1 d|, combined with the world bound t > tmin;
+//   * Moller-Trumbore in FP32, two-sided, |det| < 1e-10 * |A^-1 d| rejected (the reference's
+//     1e-10 applies to the normalised object ray);
+//   * spheres use the Ray Tracing Gems (Haines et al. 2019, ch. 7) quadratic evaluated in FP64
+//     on the FP32 ray, so large spheres (r = 1000 ground) keep sub-1e-9 hit accuracy;
+//   * equal-t ties resolve as the reference's loop order does: a later triangle of the same
+//     instance wins, an earlier instance wins over a later one, a sphere wins over instances
+//     and over earlier spheres.  Candidates carry a key; (t, key) is compared
+//     lexicographically with larger keys winning ties.
+// The traversal keeps the current node in a register and pushes only far children onto a
+// per-lane stack in LDS (stride = block size, so lanes hit distinct banks), overflowing to a
+// per-thread global area when a tree is deeper than the LDS part.
+#pragma once
+
+#include "tpt_math.h"
+
+namespace tpt {
+
+struct SceneF32 {
+  const Node2* __restrict__ nodes;
+  const TriF32* __restrict__ tris;
+  const int32_t* __restrict__ tri_mat;
+  const int32_t* __restrict__ tri_orig;
+  const InstF32* __restrict__ inst;
+  const int32_t* __restrict__ tlas_items;
+  const double* __restrict__ sph;  // c.xyz, r (FP64)
+  const int32_t* __restrict__ sph_mat;
+  int32_t tlas_root;
+  int32_t n_spheres;
+};
+
+struct HitF32 {
+  float t;
+  int32_t inst;  // instance index, TPT_ITEM_SPHERE | sphere, or -1 (miss)
+  int32_t prim;  // reordered triangle slot, -1 for spheres / miss
+};
+
+struct LaneStack {
+  int32_t* lds;     // this lane's first LDS entry
+  int32_t* glob;    // this lane's global overflow area (may be null if never needed)
+  int stride;       // LDS stride in entries (block size)
+  int cap;          // LDS entries per lane
+  TPT_HD void push(int& sp, int32_t v) const {
+    if (sp < cap) lds[sp * stride] = v;
+    else glob[sp - cap] = v;
+    ++sp;
+  }
+  TPT_HD int32_t pop(int& sp) const {
+    --sp;
+    return sp < cap ? lds[sp * stride] : glob[sp - cap];
+  }
+};
+
+TPT_HD float safe_rcp(float d) { return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d)); }
+
+#define TPT_KEY_SPHERE(s) (0x40000000 + (int32_t)(s))
+#define TPT_KEY_INST(i) (0x3fffffff - (int32_t)(i))
+#define TPT_STACK_SENTINEL TPT_REF_EMPTY
+
+// FP64 sphere root in [tmin, tmax] (near root first, then far), geometry.rs:174-195.
+TPT_HD bool sphere_hit64(const double* s, vec3<float> of, vec3<float> df, double tmin, double tmax,
+                                             double* t_out) {
+  vec3<double> o = mk((double)of.x, (double)of.y, (double)of.z), d = mk((double)df.x, (double)df.y, (double)df.z);
+  vec3<double> c = mk(s[0], s[1], s[2]);
+  double r = s[3];
+  vec3<double> f = o - c;
+  double a = dot(d, d);
+  double bp = -dot(f, d);
+  vec3<double> l = f + (bp / a) * d;
+  double disc = a * (r * r - dot(l, l));  // = b^2 - a c without cancellation
+  if (disc <= 0.0) return false;
+  double cc = dot(f, f) - r * r;
+  double q = bp + copysign(sqrt(disc), bp);
+  double ta = cc / q, tb = q / a;
+  double tn = fmin(ta, tb), tf = fmax(ta, tb);
+  double t = tn;
+  if (t < tmin || t > tmax) {
+    t = tf;
+    if (t < tmin || t > tmax) return false;
+  }
+  *t_out = t;
+  return true;
+}
+
+// Returns true on a hit.  ANY: stop at the first hit in range (shadow rays).
+template <bool ANY>
+TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmin, float tmax, HitF32* hit,
+                                          const LaneStack& st) {
+  int32_t ref = S.tlas_root;
+  hit->t = tmax;
+  hit->inst = -1;
+  hit->prim = -1;
+  if (ref == TPT_REF_EMPTY) return false;
+  const vec3<float> inv_w = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+  const vec3<float> oi_w = mk(o.x * inv_w.x, o.y * inv_w.y, o.z * inv_w.z);
+  vec3<float> co = o, cd = d, inv = inv_w, oi = oi_w;
+  float lo = tmin;        // lower t bound at the current level
+  float det_eps = 1e-10f;
+  float best_t = tmax;
+  int32_t best_key = -1;
+  int32_t cur_inst = -1;  // instance being traversed (-1: top level)
+  int32_t cur_key = 0;
+  int sp = 0;
+  bool found = false;
+  for (;;) {
+    if (ref >= 0 && ref != TPT_STACK_SENTINEL) {
+      // ---- inner node: both children's boxes from one 64 B node
+      const float4* p = reinterpret_cast<const float4*>(S.nodes + ref);
+      const float4 a = p[0], b = p[1], c = p[2];
+      const int4 m = reinterpret_cast<const int4*>(p)[3];
+      float x0 = fmaf(a.x, inv.x, -oi.x), x1 = fmaf(a.w, inv.x, -oi.x);
+      float y0 = fmaf(a.y, inv.y, -oi.y), y1 = fmaf(b.x, inv.y, -oi.y);
+      float z0 = fmaf(a.z, inv.z, -oi.z), z1 = fmaf(b.y, inv.z, -oi.z);
+      float ln = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
+      float lf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
+      x0 = fmaf(b.z, inv.x, -oi.x); x1 = fmaf(c.y, inv.x, -oi.x);
+      y0 = fmaf(b.w, inv.y, -oi.y); y1 = fmaf(c.z, inv.y, -oi.y);
+      z0 = fmaf(c.x, inv.z, -oi.z); z1 = fmaf(c.w, inv.z, -oi.z);
+      float rn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
+      float rf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
+      bool hl = ln <= lf, hr = rn <= rf;
+      if (hl && hr) {
+        // nearer first; the reference pushes far then near with `a_t0 <= b_t0` choosing left
+        bool left_first = ln <= rn;
+        ref = left_first ? m.x : m.y;
+        st.push(sp, left_first ? m.y : m.x);
+        continue;
+      }
+      if (hl) { ref = m.x; continue; }
+      if (hr) { ref = m.y; continue; }
+    } else if (ref == TPT_STACK_SENTINEL) {
+      // ---- leaving an instance: back to the world ray
+      co = o; cd = d; inv = inv_w; oi = oi_w; lo = tmin; det_eps = 1e-10f; cur_inst = -1;
+    } else if (cur_inst < 0) {
+      // ---- top-level leaf: one item
+      int32_t item = S.tlas_items[(uint32_t)ref & TPT_LEAF_START_MASK];
+      if ((uint32_t)item & TPT_ITEM_SPHERE) {
+        int32_t s = (int32_t)((uint32_t)item & ~TPT_ITEM_SPHERE);
+        double th;
+        if (sphere_hit64(S.sph + 4 * s, o, d, (double)tmin, (double)best_t, &th)) {
+          float tf = (float)th;
+          int32_t key = TPT_KEY_SPHERE(s);
+          if (tf < best_t || key >= best_key) {
+            best_t = tf; best_key = key; found = true;
+            hit->t = tf; hit->inst = (int32_t)((uint32_t)s | TPT_ITEM_SPHERE); hit->prim = -1;
+            if (ANY) return true;
+          }
+        }
+      } else {
+        const InstF32& I = S.inst[item];
+        const float4* w = reinterpret_cast<const float4*>(I.w2o);
+        const float4 r0 = w[0], r1 = w[1], r2 = w[2];
+        int32_t root = I.root;
+        if (root != TPT_REF_EMPTY) {
+          co = mk(fmaf(r0.x, o.x, fmaf(r0.y, o.y, fmaf(r0.z, o.z, r0.w))), fmaf(r1.x, o.x, fmaf(r1.y, o.y, fmaf(r1.z, o.z, r1.w))),
+                  fmaf(r2.x, o.x, fmaf(r2.y, o.y, fmaf(r2.z, o.z, r2.w))));
+          cd = mk(fmaf(r0.x, d.x, fmaf(r0.y, d.y, r0.z * d.z)), fmaf(r1.x, d.x, fmaf(r1.y, d.y, r1.z * d.z)),
+                  fmaf(r2.x, d.x, fmaf(r2.y, d.y, r2.z * d.z)));
+          float len = sqrtf(dot(cd, cd));
+          lo = fmaxf(tmin, tmin / len);
+          det_eps = 1e-10f * len;
+          inv = mk(safe_rcp(cd.x), safe_rcp(cd.y), safe_rcp(cd.z));
+          oi = mk(co.x * inv.x, co.y * inv.y, co.z * inv.z);
+          cur_inst = item;
+          cur_key = TPT_KEY_INST(item);
+          st.push(sp, TPT_STACK_SENTINEL);
+          ref = root;
+          continue;
+        }
+      }
+    } else {
+      // ---- bottom-level leaf: up to 8 triangles in consecutive slots
+      const uint32_t start = (uint32_t)ref & TPT_LEAF_START_MASK;
+      const uint32_t cnt = (((uint32_t)ref >> 28) & 7u) + 1u;
+      for (uint32_t k = 0; k < cnt; k++) {
+        const float4* tp = reinterpret_cast<const float4*>(S.tris + start + k);
+        const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
+        // Moller-Trumbore (geometry.rs:236-273)
+        vec3<float> E1 = mk(e1.x, e1.y, e1.z), E2 = mk(e2.x, e2.y, e2.z);
+        vec3<float> pvec = cross(cd, E2);
+        float det = dot(E1, pvec);
+        if (fabsf(det) < det_eps) continue;
+        float inv_det = 1.0f / det;
+        vec3<float> tvec = mk(co.x - v0.x, co.y - v0.y, co.z - v0.z);
+        float u = dot(tvec, pvec) * inv_det;
+        if (u < 0.f || u > 1.f) continue;
+        vec3<float> qvec = cross(tvec, E1);
+        float v = dot(cd, qvec) * inv_det;
+        if (v < 0.f || u + v > 1.f) continue;
+        float th = dot(E2, qvec) * inv_det;
+        if (th < lo || th > best_t) continue;
+        if (th < best_t || cur_key >= best_key) {
+          best_t = th; best_key = cur_key; found = true;
+          hit->t = th; hit->inst = cur_inst; hit->prim = (int32_t)(start + k);
+          if (ANY) return true;
+        }
+      }
+    }
+    if (sp == 0) break;
+    ref = st.pop(sp);
+  }
+  return found;
+}
+
+}  // namespace tpt

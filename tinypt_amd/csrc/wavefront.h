@@ -1,0 +1,96 @@
+// wavefront.h -- parameters of the FP32 wavefront kernels (k_wavefront.hip) and their launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tpt_kernels.h"
+#include "tpt_trace32.h"
+
+namespace tpt {
+
+#define TPT_WF_BLOCK 256
+#define TPT_WF_STACK 24  // LDS stack entries per lane (24 KB per 256-lane block)
+
+#define TPT_CNT_Q0 0
+#define TPT_CNT_Q1 1
+#define TPT_CNT_SH 2
+#define TPT_CNT_ACTIVE 3
+#define TPT_NUM_CNT 8
+
+// Path state, structure of arrays, capacity = paths in flight.
+struct PathState {
+  float4* o;     // origin xyz, pdf of the last BSDF sample
+  float4* d;     // direction xyz, flags (bits 0-7 bounce, bit 8 last bounce non-delta)
+  float4* beta;  // throughput
+  float4* L;     // radiance gathered so far
+  uint64_t* rng;
+  uint32_t* pix;
+  int4* hit;       // t bits, instance code, triangle slot
+  float4* sh_o;    // 2 per path: shadow origin, tmax
+  float4* sh_d;    // 2 per path: shadow direction
+  float4* sh_c;    // 2 per path: contribution if unoccluded, w != 0 when present
+  int32_t* q[2];   // ping-pong extension queues
+  int32_t* sq;     // paths with shadow rays this bounce
+  uint32_t* counters;
+  uint64_t* stats;  // [0] extension rays, [1] shadow rays, [2] paths
+};
+
+struct WfParams {
+  SceneF32 S;
+  EnvDev env;
+  LightsDev lights;
+  MatsDev mats;
+  CamF32 cam;
+  PathState ps;
+  int32_t* gstack;  // per-thread overflow stacks (may be null)
+  int32_t gstack_cap;
+  int32_t width, height;
+  int32_t max_bounces, rr_start;
+  float inv_w, inv_h;
+  uint64_t seed;
+};
+
+struct RaygenArgs {
+  uint32_t n_paths;   // pixels * k
+  uint32_t n_pixels;  // pixels in this pass
+  uint32_t pixel0;    // first pixel (non-adaptive)
+  uint32_t s0;        // first sample index (non-adaptive)
+  const int32_t* active;  // adaptive: active pixel list (else null)
+  const uint32_t* nsamp;  // adaptive: samples taken per pixel
+};
+
+struct AccumArgs {
+  uint32_t n_pixels, pixel0, k;
+  double* acc;
+  double* acc_w;
+  double* acc_sq;
+};
+
+struct AdaptiveArgs {
+  uint32_t n_active, k, min_spp, max_spp;
+  double threshold;
+  const int32_t* active_in;
+  int32_t* active_out;
+  uint32_t* n_active_out;
+  uint32_t* nsamp;
+  double* welford;  // mean, m2 per pixel
+  double* acc;
+  double* acc_w;
+  double* acc_sq;
+};
+
+hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s);
+hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_shadow(const WfParams& W, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s);
+hipError_t wf_launch_accumulate_adaptive(const WfParams& W, const AdaptiveArgs& A, hipStream_t s);
+uint32_t wf_grid_threads(uint32_t n);
+
+// trace_closest entry (k_trace.hip)
+hipError_t launch_trace_closest(const SceneF32& S, int64_t n, const float* rays, float* t, int32_t* kind, int32_t* inst,
+                                int32_t* prim, int32_t* gstack, int32_t gcap, hipStream_t s);
+uint32_t trace_grid_threads(int64_t n);
+
+}  // namespace tpt
