@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+Workload: the Cornell box built from rectangles and two cubes (white .73 / red .65,.05,.05 /
+green .12,.45,.15, ceiling light 17,12,4), 512x512, 256 samples per pixel, max_depth 12,
+rr_depth 4, seed 2, FP32 wavefront kernels with FP64 per-pixel accumulation.  One step = one
+full render of that image (67.1M camera paths).
+
+Printed (one JSON line):  paths/s of the FP32 wavefront render (value), Mrays/s, the plain
+FP64 GPU form's paths/s on the same workload, RMSE of both against a 2048-spp FP64 plain
+render, the roofline of the dominant kernel (extension traversal) and the FP64 CPU oracle
+timed on a bounded sample on this host.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W = H = 512
+SPP = 256
+MAX_BOUNCES = 12
+RR_START = 4
+SEED = 2
+HBM_PEAK_GBS = 8000.0
+METRIC = "paths/s & Mrays/s at fixed RMSE vs FP64 oracle; FP32 vs FP64 plain; grad s/iter"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_ray(counters):
+    """Bytes one World::hit query reads/writes by the reference traversal's work counts: 24 per
+    AABB slab test (FP32 box), 36 per triangle test, 16 per sphere test, 48 per instance
+    visit (FP32 affine inverse), 32 ray in + 16 hit out."""
+    rays = counters["rays_ext"] + counters["rays_shadow"]
+    return (24.0 * counters["aabb"] + 36.0 * counters["tri"] + 16.0 * counters["sph"] + 48.0 * counters["inst"]) / rays + 48.0
+
+
+def cpu_baseline(scene, seconds_target=15.0):
+    """The FP64 oracle (a C++ restatement of the reference CPU renderer with its scheduler:
+    16x16 tiles, Morton order, one thread per core) on a bounded sample of the workload."""
+    import oracle
+    o = oracle.OracleScene(scene)
+    threads = os.cpu_count() or 1
+    # calibrate on 1 spp, then size the sample to ~seconds_target
+    r = o.render(W, H, 1, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED, threads=threads)
+    rate = W * H / max(r["seconds"], 1e-6)
+    spp = int(max(1, min(64, seconds_target * rate / (W * H))))
+    r = o.render(W, H, spp, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED, threads=threads)
+    paths = W * H * spp
+    c = r["counters"]
+    return {"value": paths / r["seconds"], "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": f"512x512 at {spp} spp of the same scene ({paths} paths, {r['seconds']:.2f} s)",
+            "mrays_per_s": (c["rays_ext"] + c["rays_shadow"]) / r["seconds"] / 1e6}, c
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the 2048-spp FP64 reference image")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box here has one GPU).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import tinypt_amd as T
+    from tinypt_amd import scenes
+
+    scene = scenes.cornell_box(W, H)
+    ds = T.DeviceScene(scene)
+    cfg = T.RenderConfig(width=W, height=H, spp=SPP, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED,
+                         precision=T.FP32_WAVEFRONT)
+    for _ in range(args.warmup):
+        T.render(ds, cfg, want_sq=False)
+    stats = []
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(args.steps):
+        out = T.render(ds, cfg, want_sq=False)  # synchronous: returns after the device finishes
+        stats.append(out.stats)
+    t1 = time.perf_counter()
+    secs = (t1 - t0) / args.steps
+    paths = W * H * SPP
+    rays = np.mean([s["rays_extension"] + s["rays_shadow"] for s in stats])
+    value = paths / secs
+    mrays = rays / secs / 1e6
+    trace_ms = np.mean([s["trace_kernel_ms"] for s in stats])
+    trace_launches = np.mean([s["trace_launches"] for s in stats])
+    trace_rays = np.mean([s["trace_rays"] for s in stats])
+    img32 = T.resolve_pixels(out.acc, out.acc_w)
+    log(f"fp32 wavefront: {secs * 1e3:.1f} ms/step, {value / 1e6:.1f} M paths/s, {mrays:.1f} Mrays/s")
+
+    # plain FP64 GPU form on the same workload
+    cfg64 = T.RenderConfig(width=W, height=H, spp=SPP, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED,
+                           precision=T.FP64_PLAIN)
+    t0 = time.perf_counter()
+    out64 = T.render(ds, cfg64, want_sq=False)
+    secs64 = time.perf_counter() - t0
+    img64 = T.resolve_pixels(out64.acc, out64.acc_w)
+    plain_rate = paths / secs64
+    log(f"fp64 plain: {secs64 * 1e3:.1f} ms, {plain_rate / 1e6:.1f} M paths/s")
+
+    rmse = None
+    if not args.no_rmse:
+        ref_cfg = T.RenderConfig(width=W, height=H, spp=2048, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED + 1000,
+                                 precision=T.FP64_PLAIN)
+        ref = T.render(ds, ref_cfg, want_sq=False)
+        ref_img = T.resolve_pixels(ref.acc, ref.acc_w)
+        rmse = {"fp32_wavefront_256spp": float(np.sqrt(np.mean((img32 - ref_img) ** 2))),
+                "fp64_plain_256spp": float(np.sqrt(np.mean((img64 - ref_img) ** 2))),
+                "reference": "FP64 plain GPU render, 2048 spp, independent seed"}
+        log(f"rmse: {rmse}")
+
+    cpu = None
+    bytes_per_ray = None
+    if not args.no_cpu:
+        cpu, counters = cpu_baseline(scene)
+        bytes_per_ray = algorithmic_bytes_per_ray(counters)
+        log(f"cpu oracle: {cpu}")
+    else:
+        import oracle
+        r = oracle.OracleScene(scene).render(64, 64, 16, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED)
+        bytes_per_ray = algorithmic_bytes_per_ray(r["counters"])
+
+    avg_launch_ms = trace_ms / max(trace_launches, 1)
+    rays_per_launch = trace_rays / max(trace_launches, 1)
+    achieved = bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "k_extend (FP32 closest-hit traversal)",
+                "algorithmic_bytes_per_ray": bytes_per_ray, "avg_launch_ms": avg_launch_ms,
+                "rays_per_launch": rays_per_launch}
+
+    line = {"metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": secs * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (scene generated in code)",
+            "config": {"workload": "cornell_box_512x512_256spp_maxdepth12_rrdepth4_seed2", "width": W, "height": H,
+                       "spp": SPP, "max_depth": MAX_BOUNCES, "rr_depth": RR_START, "paths_per_step": paths,
+                       "accumulation": "fp64"},
+            "mrays_per_s": mrays, "fp64_plain_paths_per_s": plain_rate, "speedup_vs_fp64_plain": value / plain_rate,
+            "rmse": rmse, "roofline": roofline, "cpu_baseline": cpu}
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

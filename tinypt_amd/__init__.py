@@ -27,7 +27,7 @@ from .scene import (Camera, DiffuseLight, Dielectric, EnvMap, Ggx, Instance, Lam
 __all__ = ["Camera", "DiffuseLight", "Dielectric", "EnvMap", "Ggx", "Instance", "Lambert", "Material", "Mesh", "Metal",
            "RenderConfig", "Scene", "Sphere", "Subsurface", "Transform", "World", "Context", "DeviceScene", "HostScene",
            "RenderOutput",
-           "render", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
+           "render", "render_adjoint", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
 
 import atexit
 import weakref
@@ -252,6 +252,26 @@ def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Cont
     _check(lib.tpt_render(dscene.ctx.h, dscene.h, C.byref(cfg), _dp(acc), _dp(acc_w),
                           _dp(acc_sq) if want_sq else None, C.byref(st)))
     return RenderOutput(acc, acc_w, acc_sq, st.as_dict())
+
+
+def render_adjoint(scene: "DeviceScene", config: RenderConfig, d_loss_d_pixel: np.ndarray,
+                   grad: Optional[np.ndarray] = None) -> tuple:
+    """Gradient of a loss with respect to every material's 4 parameters (``tpt_render_adjoint``).
+
+    ``d_loss_d_pixel`` (h, w, 3) is dLoss/dpixel of the resolved image (acc / spp).  The render
+    is the FP32 wavefront estimator with the same seeds, replayed path by path; the result is
+    added into ``grad`` (n_materials, 4), which is returned together with the run counters."""
+    lib = scene._lib
+    h, w = int(config.height), int(config.width)
+    dl = np.ascontiguousarray(np.asarray(d_loss_d_pixel, dtype=np.float32).reshape(h, w, 3))
+    g = np.zeros((scene.n_materials, 4)) if grad is None else grad
+    if g.dtype != np.float64 or not g.flags.c_contiguous or g.shape != (scene.n_materials, 4):
+        raise ValueError("grad must be a C-contiguous float64 array of shape (n_materials, 4)")
+    st = _abi.Stats()
+    cfg = config.to_c()
+    _check(lib.tpt_render_adjoint(scene.ctx.h, scene.h, C.byref(cfg), dl.ctypes.data_as(C.POINTER(C.c_float)), _dp(g),
+                                  C.byref(st)))
+    return g, st.as_dict()
 
 
 def resolve_pixels(acc: np.ndarray, acc_w: np.ndarray) -> np.ndarray:

@@ -406,6 +406,7 @@ int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
 WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, int gcap) {
   WfParams W{};
   W.S = sc->s32;
+  W.S64 = sc->s64;
   W.env = sc->env;
   W.lights = sc->lights;
   W.mats = sc->mats;
@@ -720,7 +721,7 @@ int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_co
   int rc = check_cfg(cfg);
   if (rc) return rc;
   HIPC(hipSetDevice(ctx->device));
-  return adjoint_render(ctx->stream, scene->s32, scene->env, scene->lights, scene->mats, scene->cam32,
+  return adjoint_render(ctx->stream, scene->s32, scene->s64, scene->env, scene->lights, scene->mats, scene->cam32,
                         needed_gstack(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
 }
 

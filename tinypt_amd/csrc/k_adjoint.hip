@@ -1,12 +1,421 @@
-// k_adjoint.hip -- path-replay adjoint (placeholder until the replay kernels land).
+// k_adjoint.hip -- path-replay adjoint of the FP32 render (gradients of a pixel loss with respect
+// to material parameters).
+//
+// Every (pixel, sample) path is regenerated from its seed, seed_for(x, y, s) ^ seed, with the
+// same random-number consumption as the forward wavefront kernels, so the adjoint sees exactly
+// the paths the forward render summed.  Nothing is stored per path: each path is walked twice.
+//   Pass A sums, over the path's radiance terms c_k (misses, emitter hits, unoccluded NEE),
+//     s_k = x_k o (delta^T J_k), where x_k is the term before the firefly clamp, J_k the clamp's
+//     Jacobian (identity when not clamped) and delta = dLoss/dPixel / spp.
+//   Pass B walks the path again.  A term's own parameters (BSDF value and MIS weight of its
+//     vertex, emitter radiance) are differentiated where the term is formed; a BSDF sample's
+//     weight w_j scales every later term, so its parameters receive
+//     sum_c (dw_jc/dtheta / w_jc) * (S_total - S_upto_j)_c, the suffix of the pass-A sum.
+// Sampling pdfs (BSDF, light selection, env) and Russian-roulette probabilities are held fixed:
+// the estimator divides by them, so differentiating the integrand alone gives the derivative
+// of the expected image.  Parameters: per material 4 slots laid out like mat_param -- Lambert /
+// Metal / Subsurface albedo rgb, GGX F0 rgb + alpha, Dielectric absorption rgb, DiffuseLight
+// emit rgb.
+#include <string>
+
 #include "adjoint.h"
+#include "tpt_sample32.h"
+#include "tpt_selfocc64.h"
+#include "wavefront.h"
 
 namespace tpt {
 
-int adjoint_render(hipStream_t, const SceneF32&, const EnvDev&, const LightsDev&, const MatsDev&, const CamF32&, int,
-                   const tpt_render_config*, const float*, double*, tpt_stats*, std::string* err) {
-  *err = "adjoint not built yet";
-  return TPT_ERR_UNSUPPORTED;
+#define TPT_ADJ_BLOCK 256
+#define TPT_ADJ_LDS_GRAD 1024  // per-block gradient slots in LDS (256 materials)
+
+struct AdjParams {
+  SceneF32 S;
+  SceneF64 S64;  // reference-layout scene, for the light self-occlusion decision
+  EnvDev env;
+  LightsDev lights;
+  MatsDev mats;
+  CamF32 cam;
+  int32_t* gstack;
+  int32_t gcap;
+  int32_t width, height, spp, max_bounces, rr_start, n_mats;
+  float inv_w, inv_h;
+  uint64_t seed;
+  uint64_t n_paths;
+  const float* dldp;  // w*h*3
+  double* grad;       // n_mats*4
+  uint64_t* stats;    // [0] extension rays, [1] shadow rays, [2] paths
+};
+
+// Firefly clamp (math.rs:42-49) and v^T J for its Jacobian.
+__device__ __forceinline__ f3 clamp_vjp(f3 x, f3 v) {
+  const float m = (float)TPT_FIREFLY_CLAMP;
+  float l = luminance(x);
+  if (!(l > m && l > 0.f)) return v;
+  f3 lam = mk(0.2126f, 0.7152f, 0.0722f);
+  float vx = dot(v, x);
+  return (m / l) * v - (m * vx / (l * l)) * lam;
+}
+
+// GGX helpers: derivatives with respect to alpha (material.rs:272-293).
+__device__ __forceinline__ float d_ggx_distribution_da(float a, float c) {
+  if (c <= 0.f) return 0.f;
+  float a2 = a * a;
+  float k = c * c * (a2 - 1.f) + 1.f;
+  return 2.f * a * (k - 2.f * a2 * c * c) / ((float)TPT_PI * k * k * k);
+}
+__device__ __forceinline__ float d_ggx_g1_da(float a, float c) {
+  if (c <= 0.f) return 0.f;
+  float t = sqrtf(fmaxf(1.f - c * c, 0.f)) / fmaxf(c, 1e-6f);
+  float x = a * t;
+  float r = sqrtf(1.f + x * x);
+  return -2.f * (x / r) / ((1.f + r) * (1.f + r)) * t;
+}
+
+// d/d(F0, alpha) of ggx_eval and ggx_pdf at (n, wo, wi).  f_dF0: per channel dF_c/dF0_c * D G /
+// denom; f_da: dF/d alpha per channel; pdf_da: d pdf / d alpha.
+struct GgxGrad {
+  f3 f_dF0;
+  f3 f_da;
+  float pdf_da;
+};
+__device__ __forceinline__ GgxGrad ggx_grad(f3 albedo, float s, f3 n, f3 wo, f3 wi) {
+  GgxGrad g{mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), 0.f};
+  float alpha = fmaxf(s, 1e-3f);
+  float da_ds = s > 1e-3f ? 1.f : 0.f;
+  float ci = dot(n, wo), co = dot(n, wi);
+  if (ci <= 0.f || co <= 0.f) return g;
+  f3 m = vnorm(wo + wi);
+  float ch = fmaxf(dot(n, m), 0.f);
+  float D = ggx_distribution(alpha, ch), dD = d_ggx_distribution_da(alpha, ch);
+  float G1i = ggx_smith_g1(alpha, ci), G1o = ggx_smith_g1(alpha, co);
+  float dG1i = d_ggx_g1_da(alpha, ci), dG1o = d_ggx_g1_da(alpha, co);
+  float G = G1i * G1o, dG = dG1i * G1o + G1i * dG1o;
+  float denom = 4.f * ci * co + 1e-6f;
+  float p5 = pow5(1.f - fmaxf(fabsf(dot(wi, m)), 0.f));
+  f3 F = fresnel_schlick(fabsf(dot(wi, m)), albedo);
+  g.f_dF0 = mk(1.f - p5, 1.f - p5, 1.f - p5) * (D * G / denom);
+  g.f_da = F * ((dD * G + D * dG) / denom * da_ds);
+  float pdf_m = D * G1i * ch / fmaxf(ci, 1e-6f);
+  float pden = 4.f * fmaxf(fabsf(dot(wi, m)), 1e-6f);
+  if (pdf_m / pden > 0.f) g.pdf_da = (dD * G1i + D * dG1i) * ch / fmaxf(ci, 1e-6f) / pden * da_ds;
+  return g;
+}
+__device__ __forceinline__ float dmis_dpb(float pa, float pb) {  // d/dpb of pa^2 / (pa^2 + pb^2)
+  float a2 = pa * pa, b2 = pb * pb, s = a2 + b2;
+  return s > 0.f ? -2.f * a2 * pb / (s * s) : 0.f;
+}
+
+struct GradSink {
+  float* lds;  // block slots (or null)
+  double* global;
+  int n;
+  __device__ __forceinline__ void add(int mat, int k, float v) {
+    if (v == 0.f || !(v == v)) return;
+    int i = 4 * mat + k;
+    if (lds && i < TPT_ADJ_LDS_GRAD) atomicAdd(lds + i, v);
+    else atomicAdd(global + i, (double)v);
+  }
+};
+
+// One path.  GRAD = false: returns S_total.  GRAD = true: accumulates gradients given S_total.
+template <bool GRAD>
+__device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 delta, f3 S_total, GradSink& sink,
+                     const LaneStack& st, uint32_t* n_ext, uint32_t* n_sh) {
+  RngF32 rng(seed_for(x, y, s) ^ A.seed);
+  f3 o, d;
+  camera_ray32(A.cam, x, y, A.inv_w, A.inv_h, rng, &o, &d);
+  f3 T = mk(1.f, 1.f, 1.f), S_upto = mk(0.f, 0.f, 0.f);
+  float last_pdf = 0.f, last_dpdf = 0.f;
+  bool last_nd = false;
+  int last_mat = -1;
+  const bool env_img = A.env.kind == TPT_ENV_IMAGE;
+  for (int bounce = 0; bounce < A.max_bounces; bounce++) {
+    HitF32 h;
+    (*n_ext)++;
+    if (!trace_f32<false>(A.S, o, d, (float)TPT_RAY_EPSILON, 1e30f, &h, st)) {
+      // miss (integrator.rs:71-83)
+      f3 bg = env_img ? env_lookup32(A.env, d) : sky32(d);
+      f3 x0 = had(T, bg);
+      float w = 1.f, pe = 0.f;
+      if (env_img && last_nd) {
+        pe = env_pdf32(A.env, d);
+        w = mis_weight(last_pdf, pe);
+      }
+      f3 xw = x0 * w;
+      f3 vj = clamp_vjp(xw, delta);
+      S_upto = S_upto + had(xw, vj);
+      if (GRAD && env_img && last_nd && last_mat >= 0 && A.mats.kind[last_mat] == TPT_MAT_GGX)
+        sink.add(last_mat, 3, -dot(vj, x0) * dmis_dpb(pe, last_pdf) * last_dpdf);  // d/d(first arg)
+      break;
+    }
+    if (bounce >= A.rr_start) {  // integrator.rs:89-95 (probability held fixed)
+      float p = fmaxf(fminf(fmaxf(fmaxf(T.x, T.y), T.z), 0.95f), 0.05f);
+      if (rng.uni() > p) break;
+      T = T / p;
+    }
+    f3 p, ng;
+    int mat;
+    hit_geometry32(A.S, h.inst, h.prim, h.t, o, d, &p, &ng, &mat);
+    const int kind = A.mats.kind[mat];
+    const float4 mp = A.mats.p32[mat];
+    const f3 mc = f3of(mp);
+    if (kind == TPT_MAT_DIFFUSE_LIGHT) {  // integrator.rs:100-104
+      f3 x0 = had(T, mc);
+      f3 vj = clamp_vjp(x0, delta);
+      S_upto = S_upto + had(x0, vj);
+      if (GRAD) {
+        sink.add(mat, 0, vj.x * T.x);
+        sink.add(mat, 1, vj.y * T.y);
+        sink.add(mat, 2, vj.z * T.z);
+      }
+      break;
+    }
+    f3 n = dot(ng, d) < 0.f ? ng : -ng;
+    f3 wo = vnorm(-d);
+    if (!mat_is_delta(kind)) {
+      if (env_img) {  // nee_environment (integrator.rs:148-172)
+        f3 wi, li;
+        float pe;
+        env_sample32(A.env, rng, &wi, &li, &pe);
+        float cs = fmaxf(dot(n, wi), 0.f);
+        if (cs > 0.f && pe > 0.f && (li.x != 0.f || li.y != 0.f || li.z != 0.f)) {
+          f3 f;
+          float pb;
+          bsdf_eval(kind, mc, mp.w, wo, wi, n, &f, &pb);
+          float w = mis_weight(pe, pb);
+          f3 c = had(had(T, f), li) * (cs * w / pe);
+          if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
+            HitF32 hs;
+            (*n_sh)++;
+            if (!trace_f32<true>(A.S, p + (float)TPT_RAY_EPSILON * wi, wi, (float)TPT_RAY_EPSILON, 1e30f, &hs, st)) {
+              S_upto = S_upto + had(c, delta);
+              if (GRAD) {
+                f3 base = had(had(T, li), delta) * (cs / pe);  // dc_c/df_c * delta_c / w
+                if (kind == TPT_MAT_LAMBERT || kind == TPT_MAT_SUBSURFACE) {
+                  sink.add(mat, 0, base.x * w / (float)TPT_PI);
+                  sink.add(mat, 1, base.y * w / (float)TPT_PI);
+                  sink.add(mat, 2, base.z * w / (float)TPT_PI);
+                } else if (kind == TPT_MAT_GGX) {
+                  GgxGrad g = ggx_grad(mc, mp.w, n, wo, wi);
+                  sink.add(mat, 0, base.x * w * g.f_dF0.x);
+                  sink.add(mat, 1, base.y * w * g.f_dF0.y);
+                  sink.add(mat, 2, base.z * w * g.f_dF0.z);
+                  sink.add(mat, 3, dot(base, g.f_da) * w + dot(base, f) * dmis_dpb(pe, pb) * g.pdf_da);
+                }
+              }
+            }
+          }
+        }
+      }
+      LightSample32 ls;  // nee_area_light (integrator.rs:178-213)
+      if (sample_light32(A.lights, rng, p, &ls)) {
+        f3 to_light = ls.pos - p;
+        float dist = sqrtf(dot(to_light, to_light));
+        if (dist > 1e-6f) {
+          f3 wi = to_light / dist;
+          float cs = fmaxf(dot(n, wi), 0.f);
+          if (cs > 0.f) {
+            f3 f;
+            float pb;
+            bsdf_eval(kind, mc, mp.w, wo, wi, n, &f, &pb);
+            float w = mis_weight(ls.pdf, pb);
+            f3 c = had(had(T, f), ls.emit) * (cs * w / ls.pdf);
+            if ((c.x != 0.f || c.y != 0.f || c.z != 0.f) &&
+                !light_self_occluded64(A.S64, A.lights, ls.idx, p.x, p.y, p.z, ls.u, ls.v)) {
+              float tmax = light_shadow_tmax32(dist);
+              HitF32 hs;
+              (*n_sh)++;
+              if (!trace_f32<true>(A.S, p + (float)TPT_RAY_EPSILON * wi, wi, (float)TPT_RAY_EPSILON, tmax, &hs, st)) {
+                S_upto = S_upto + had(c, delta);
+                if (GRAD) {
+                  f3 base = had(had(T, ls.emit), delta) * (cs / ls.pdf);
+                  if (kind == TPT_MAT_LAMBERT || kind == TPT_MAT_SUBSURFACE) {
+                    sink.add(mat, 0, base.x * w / (float)TPT_PI);
+                    sink.add(mat, 1, base.y * w / (float)TPT_PI);
+                    sink.add(mat, 2, base.z * w / (float)TPT_PI);
+                  } else if (kind == TPT_MAT_GGX) {
+                    GgxGrad g = ggx_grad(mc, mp.w, n, wo, wi);
+                    sink.add(mat, 0, base.x * w * g.f_dF0.x);
+                    sink.add(mat, 1, base.y * w * g.f_dF0.y);
+                    sink.add(mat, 2, base.z * w * g.f_dF0.z);
+                    sink.add(mat, 3, dot(base, g.f_da) * w + dot(base, f) * dmis_dpb(ls.pdf, pb) * g.pdf_da);
+                  }
+                  // emitter radiance of the sampled light
+                  f3 e = had(had(T, f), delta) * (cs * w / ls.pdf);
+                  sink.add(ls.mat, 0, e.x);
+                  sink.add(ls.mat, 1, e.y);
+                  sink.add(ls.mat, 2, e.z);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (bounce + 1 >= A.max_bounces) break;
+    BsdfSample<float> bs;
+    if (!bsdf_sample(kind, mc, mp.w, d, p, ng, h.t, rng, &bs)) break;
+    if (GRAD) {
+      // the sampled weight scales every later term: d log w / d theta times the suffix sum
+      f3 suf = S_total - S_upto;
+      if (kind == TPT_MAT_LAMBERT || kind == TPT_MAT_METAL || kind == TPT_MAT_SUBSURFACE) {
+        if (mc.x > 0.f) sink.add(mat, 0, suf.x / mc.x);
+        if (mc.y > 0.f) sink.add(mat, 1, suf.y / mc.y);
+        if (mc.z > 0.f) sink.add(mat, 2, suf.z / mc.z);
+      } else if (kind == TPT_MAT_GGX) {
+        f3 wi = bs.d;
+        float alpha = fmaxf(mp.w, 1e-3f);
+        float ci = dot(n, wo), co = dot(n, wi);
+        f3 m = vnorm(wo + wi);
+        float ch = fmaxf(dot(n, m), 0.f);
+        float D = ggx_distribution(alpha, ch);
+        float G = ggx_smith(alpha, ci, co);
+        GgxGrad g = ggx_grad(mc, mp.w, n, wo, wi);
+        f3 F = fresnel_schlick(fabsf(dot(wi, m)), mc);
+        float denom = 4.f * ci * co + 1e-6f;
+        float dg = D * G / denom;
+        if (dg > 0.f) {
+          if (F.x > 0.f) sink.add(mat, 0, suf.x * g.f_dF0.x / (F.x * dg));
+          if (F.y > 0.f) sink.add(mat, 1, suf.y * g.f_dF0.y / (F.y * dg));
+          if (F.z > 0.f) sink.add(mat, 2, suf.z * g.f_dF0.z / (F.z * dg));
+          // dlog(D G)/dalpha is the same for all channels
+          float dlog = (g.f_da.x / fmaxf(F.x, 1e-30f)) / dg;
+          sink.add(mat, 3, (suf.x + suf.y + suf.z) * dlog);
+        }
+        last_dpdf = g.pdf_da;
+      } else if (kind == TPT_MAT_DIELECTRIC && dot(ng, d) >= 0.f) {
+        // exiting: weight carries exp(-sigma_c t) (material.rs:111-119)
+        float dist = fmaxf(h.t, 0.f);
+        sink.add(mat, 0, -dist * suf.x);
+        sink.add(mat, 1, -dist * suf.y);
+        sink.add(mat, 2, -dist * suf.z);
+      }
+    }
+    T = had(T, bs.weight);
+    o = bs.o;
+    d = bs.d;
+    last_pdf = bs.pdf;
+    last_nd = !bs.is_delta;
+    last_mat = mat;
+  }
+  return S_upto;
+}
+
+__global__ void __launch_bounds__(TPT_ADJ_BLOCK) k_adjoint(AdjParams A) {
+  __shared__ int32_t s_stack[TPT_WF_STACK_ADJ * TPT_ADJ_BLOCK];
+  __shared__ float s_grad[TPT_ADJ_LDS_GRAD];
+  for (int i = threadIdx.x; i < TPT_ADJ_LDS_GRAD; i += blockDim.x) s_grad[i] = 0.f;
+  __syncthreads();
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  LaneStack st{s_stack + threadIdx.x, A.gstack ? A.gstack + gid * (uint64_t)A.gcap : nullptr, TPT_ADJ_BLOCK,
+               TPT_WF_STACK_ADJ};
+  GradSink sink{s_grad, A.grad, A.n_mats};
+  const uint32_t npix = (uint32_t)A.width * (uint32_t)A.height;
+  uint32_t n_ext = 0, n_sh = 0, n_paths = 0;
+  for (uint64_t i = gid; i < A.n_paths; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t pix = (uint32_t)(i % npix), s = (uint32_t)(i / npix);
+    uint32_t x = pix % (uint32_t)A.width, y = pix / (uint32_t)A.width;
+    f3 delta = mk(A.dldp[3 * pix], A.dldp[3 * pix + 1], A.dldp[3 * pix + 2]) * (1.f / (float)A.spp);
+    if (delta.x == 0.f && delta.y == 0.f && delta.z == 0.f) continue;
+    f3 S_total = replay<false>(A, x, y, s, delta, mk(0.f, 0.f, 0.f), sink, st, &n_ext, &n_sh);
+    replay<true>(A, x, y, s, delta, S_total, sink, st, &n_ext, &n_sh);
+    n_paths++;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4 * A.n_mats && i < TPT_ADJ_LDS_GRAD; i += blockDim.x)
+    if (s_grad[i] != 0.f) atomicAdd(A.grad + i, (double)s_grad[i]);
+  unsigned long long e = n_ext, sh = n_sh, pa = n_paths;
+  for (int off = 32; off > 0; off >>= 1) {
+    e += (unsigned long long)__shfl_down((long long)e, off);
+    sh += (unsigned long long)__shfl_down((long long)sh, off);
+    pa += (unsigned long long)__shfl_down((long long)pa, off);
+  }
+  if (__lane_id() == 0) {
+    if (e) atomicAdd((unsigned long long*)&A.stats[0], e);
+    if (sh) atomicAdd((unsigned long long*)&A.stats[1], sh);
+    if (pa) atomicAdd((unsigned long long*)&A.stats[2], pa);
+  }
+}
+
+int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const EnvDev& env, const LightsDev& lights, const MatsDev& mats,
+                   const CamF32& cam, int gstack_need, const tpt_render_config* cfg, const float* d_loss_d_pixel,
+                   double* grad_params, tpt_stats* stats, std::string* err) {
+  auto fail = [&](int code, const std::string& m) {
+    *err = m;
+    return code;
+  };
+  if (cfg->adaptive_enabled) return fail(TPT_ERR_UNSUPPORTED, "the adjoint replays fixed-spp renders only");
+  const size_t npix = (size_t)cfg->width * cfg->height;
+  const uint64_t n_paths = (uint64_t)npix * (uint64_t)std::max(cfg->spp, 0);
+  const int blocks = (int)std::min<uint64_t>((n_paths + TPT_ADJ_BLOCK - 1) / TPT_ADJ_BLOCK, 8192);
+  const int gcap = gstack_need > 0 ? gstack_need + TPT_WF_STACK - TPT_WF_STACK_ADJ : 0;
+  void *d_dldp = nullptr, *d_grad = nullptr, *d_stats = nullptr, *d_gs = nullptr;
+  auto cleanup = [&]() {
+    if (d_dldp) (void)hipFree(d_dldp);
+    if (d_grad) (void)hipFree(d_grad);
+    if (d_stats) (void)hipFree(d_stats);
+    if (d_gs) (void)hipFree(d_gs);
+  };
+  hipError_t e = hipMalloc(&d_dldp, npix * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_grad, (size_t)mats.n * 4 * 8 + 8);
+  if (e == hipSuccess) e = hipMalloc(&d_stats, 64);
+  if (e == hipSuccess && gcap > 0) e = hipMalloc(&d_gs, (size_t)std::max(blocks, 1) * TPT_ADJ_BLOCK * gcap * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_dldp, d_loss_d_pixel, npix * 12, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_grad, grad_params, (size_t)mats.n * 4 * 8, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d_stats, 0, 64, s);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(TPT_ERR_DEVICE, std::string("adjoint setup: ") + hipGetErrorString(e));
+  }
+  AdjParams A{};
+  A.S = S;
+  A.S64 = S64;
+  A.env = env;
+  A.lights = lights;
+  A.mats = mats;
+  A.cam = cam;
+  A.gstack = (int32_t*)d_gs;
+  A.gcap = gcap;
+  A.width = cfg->width;
+  A.height = cfg->height;
+  A.spp = cfg->spp;
+  A.max_bounces = cfg->max_bounces;
+  A.rr_start = cfg->rr_start;
+  A.n_mats = mats.n;
+  A.inv_w = (float)(1.0 / (double)cfg->width);
+  A.inv_h = (float)(1.0 / (double)cfg->height);
+  A.seed = cfg->seed;
+  A.n_paths = n_paths;
+  A.dldp = (const float*)d_dldp;
+  A.grad = (double*)d_grad;
+  A.stats = (uint64_t*)d_stats;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, s);
+  if (n_paths > 0) hipLaunchKernelGGL(k_adjoint, dim3(std::max(blocks, 1)), dim3(TPT_ADJ_BLOCK), 0, s, A);
+  e = hipGetLastError();
+  (void)hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(grad_params, d_grad, (size_t)mats.n * 4 * 8, hipMemcpyDeviceToHost, s);
+  uint64_t h[8] = {0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, 64, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cleanup();
+  if (e != hipSuccess) return fail(TPT_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
+  if (stats) {
+    stats->paths = h[2];
+    stats->rays_extension = h[0];
+    stats->rays_shadow = h[1];
+    stats->elapsed_ms = ms;
+    stats->trace_kernel_ms = ms;
+    stats->trace_launches = 1;
+    stats->trace_rays = h[0] + h[1];
+  }
+  return TPT_OK;
 }
 
 }  // namespace tpt

@@ -11,14 +11,12 @@
 // Random numbers: one PCG32 stream per (pixel, sample): seed_for(x, y, s) ^ seed.
 #include "tpt_bsdf.h"
 #include "tpt_kernels.h"
+#include "tpt_sample32.h"
+#include "tpt_selfocc64.h"
 #include "tpt_trace32.h"
 #include "wavefront.h"
 
 namespace tpt {
-
-typedef vec3<float> f3;
-
-__device__ __forceinline__ f3 f3of(float4 a) { return mk(a.x, a.y, a.z); }
 
 // ---------------------------------------------------------------- queue compaction
 __device__ __forceinline__ void wave_enqueue(bool pred, int32_t val, int32_t* __restrict__ q, uint32_t* __restrict__ count) {
@@ -30,121 +28,6 @@ __device__ __forceinline__ void wave_enqueue(bool pred, int32_t val, int32_t* __
   if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
   base = (uint32_t)__shfl((int)base, (int)leader);
   if (pred) q[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = val;
-}
-
-// ---------------------------------------------------------------- environment (env.rs), FP32
-__device__ __forceinline__ f3 sky32(f3 d) {  // integrator.rs:25-28
-  float t = 0.5f * (d.y + 1.0f);
-  return (1.0f - t) * mk(0.9f, 0.95f, 1.0f) + t * mk(0.6f, 0.75f, 1.0f);
-}
-__device__ __forceinline__ void dir_uv(f3 dir, float* u, float* v, float* theta) {
-  f3 d = vnorm(dir);
-  float th = acosf(fminf(fmaxf(d.y, -1.f), 1.f));
-  float phi = atan2f(d.z, d.x);
-  if (phi < 0.f) phi += (float)TPT_TAU;
-  *u = phi / (float)TPT_TAU;
-  *v = th / (float)TPT_PI;
-  *theta = th;
-}
-// EnvMap::sample (env.rs:92-121): bilinear, x = u W (wraps), y = v (H - 1).
-__device__ __forceinline__ f3 env_lookup32(const EnvDev& E, f3 dir) {
-  float u, v, th;
-  dir_uv(dir, &u, &v, &th);
-  float x = u * (float)E.w;
-  float y = v * ((float)E.h - 1.f);
-  float fxf = floorf(x);
-  int x0 = ((int)fxf) % E.w;
-  if (x0 < 0) x0 += E.w;
-  int x1 = (x0 + 1) % E.w;
-  int y0 = (int)floorf(fminf(fmaxf(y, 0.f), (float)(E.h - 1)));
-  int y1 = min(y0 + 1, E.h - 1);
-  float fx = x - fxf, fy = y - floorf(y);
-  float4 c00 = E.rgb32[y0 * E.w + x0], c10 = E.rgb32[y0 * E.w + x1];
-  float4 c01 = E.rgb32[y1 * E.w + x0], c11 = E.rgb32[y1 * E.w + x1];
-  f3 c0 = f3of(c00) * (1.f - fx) + f3of(c10) * fx;
-  f3 c1 = f3of(c01) * (1.f - fx) + f3of(c11) * fx;
-  return c0 * (1.f - fy) + c1 * fy;
-}
-// EnvMap::pdf (env.rs:156-182) with the per-cell numerator precomputed in FP64.
-__device__ __forceinline__ float env_pdf32(const EnvDev& E, f3 dir) {
-  if (!(E.total > 0.0)) return 1.0f / (4.0f * (float)TPT_PI);
-  float u, v, th;
-  dir_uv(dir, &u, &v, &th);
-  int x = (int)fminf(fmaxf(floorf(u * (float)E.w), 0.f), (float)(E.w - 1));
-  int y = (int)fminf(fmaxf(floorf(v * (float)E.h), 0.f), (float)(E.h - 1));
-  return E.cell_pdf[y * E.w + x] / fmaxf(sinf(th), 1e-6f);
-}
-// EnvMap::sample_dir (env.rs:124-152): the cell is drawn from the alias table, which realises
-// the same cell probabilities weight/total as the reference's row/column CDFs.
-__device__ __forceinline__ void env_sample32(const EnvDev& E, RngF32& rng, f3* dir, f3* li, float* pdf) {
-  if (!(E.total > 0.0)) {  // sample_uniform_env (env.rs:200-210)
-    float u = rng.uni(), v = rng.uni();
-    float z = 1.f - 2.f * u;
-    float r = sqrtf(fmaxf(1.f - z * z, 0.f));
-    float phi = (float)TPT_TAU * v;
-    *dir = mk(r * cosf(phi), z, r * sinf(phi));
-    *li = env_lookup32(E, *dir);
-    *pdf = 1.0f / (4.0f * (float)TPT_PI);
-    return;
-  }
-  const int N = E.w * E.h;
-  int cell = min((int)(rng.uni() * (float)N), N - 1);
-  if (rng.uni() >= E.alias_prob[cell]) cell = E.alias_idx[cell];
-  int x = cell % E.w, y = cell / E.w;
-  float u = ((float)x + rng.uni()) / (float)E.w;
-  float v = ((float)y + rng.uni()) / (float)E.h;
-  float theta = (float)TPT_PI * v, phi = (float)TPT_TAU * u;
-  float st = sinf(theta);
-  *dir = mk(cosf(phi) * st, cosf(theta), sinf(phi) * st);
-  *li = env_lookup32(E, *dir);
-  *pdf = env_pdf32(E, *dir);
-}
-
-// ---------------------------------------------------------------- lights (world.rs:171-272), FP32
-struct LightSample32 {
-  f3 pos, emit;
-  float pdf;
-};
-__device__ __forceinline__ bool sample_light32(const LightsDev& Ls, RngF32& rng, f3 p, LightSample32* out) {
-  if (!(Ls.total32 > 0.f) || Ls.n == 0) return false;
-  float r = rng.uni() * Ls.total32;
-  int idx = min(cdf_search(Ls.cdf32, Ls.n, r), Ls.n - 1);
-  const LightF32& L = Ls.l32[idx];
-  float pdf_select = L.emit[3] / Ls.total32;
-  f3 pos, nrm;
-  float area;
-  float u = rng.uni(), v = rng.uni();
-  if (L.kind == 0) {
-    float z = 1.f - 2.f * u;
-    float rr = sqrtf(fmaxf(1.f - z * z, 0.f));
-    float phi = (float)TPT_TAU * v;
-    f3 dir = mk(rr * cosf(phi), z, rr * sinf(phi));
-    pos = ld3(L.p0) + dir * L.p0[3];
-    nrm = dir;
-    area = 4.f * (float)TPT_PI * L.p0[3] * L.p0[3];
-  } else {
-    f3 a = ld3(L.p0), b = ld3(L.p1), c = ld3(L.p2);
-    float su = sqrtf(u);
-    float b0 = 1.f - su, b1 = v * su, b2 = 1.f - b0 - b1;
-    pos = a * b0 + b * b1 + c * b2;
-    f3 n = cross(b - a, c - a);
-    area = 0.5f * vlen(n);
-    nrm = area > 0.f ? n / (2.f * area) : mk(0.f, 1.f, 0.f);
-  }
-  if (area <= 0.f) return false;
-  f3 to_light = pos - p;
-  float dist2 = dot(to_light, to_light);
-  if (dist2 <= 1e-12f) return false;
-  float dist = sqrtf(dist2);
-  f3 wi = to_light / dist;
-  float cos_light = fmaxf(dot(nrm, -wi), 0.f);
-  if (cos_light <= 0.f) return false;
-  float pdf = pdf_select * ((1.f / area) * dist2 / cos_light);
-  if (!(pdf > 0.f)) return false;
-  out->pos = pos;
-  out->emit = mk(L.emit[0], L.emit[1], L.emit[2]);
-  out->pdf = pdf;
-  return true;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -164,24 +47,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
     }
     uint32_t x = pix % (uint32_t)W.width, y = pix / (uint32_t)W.width;
     RngF32 rng(seed_for(x, y, s) ^ W.seed);
-    // render.rs:153-157 jitter and Camera::ray (ray.rs:91-108); the shutter time is unused by
-    // static geometry and is not drawn.
-    float jx = rng.uni(), jy = rng.uni();
-    float sx = ((float)x + jx) * W.inv_w * 2.f - 1.f;
-    float sy = 1.f - ((float)y + jy) * W.inv_h * 2.f;
-    const CamF32& C = W.cam;
-    f3 dir_base = vnorm(-ld3(C.w) + (sx * C.half_w) * ld3(C.u) + (sy * C.half_h) * ld3(C.v));
-    f3 origin = ld3(C.o);
-    f3 dir = dir_base;
-    if (C.lens_radius > 0.f) {
-      float dx = 0.f, dy = 0.f;
-      for (int tries = 0; tries < 256; tries++) {  // sample_unit_disk (ray.rs:112-120)
-        float a = 2.f * rng.uni() - 1.f, b = 2.f * rng.uni() - 1.f;
-        if (a * a + b * b < 1.f) { dx = a; dy = b; break; }
-      }
-      origin = origin + ld3(C.u) * (dx * C.lens_radius) + ld3(C.v) * (dy * C.lens_radius);
-      dir = vnorm((ld3(C.o) + dir_base * C.focus_dist) - origin);
-    }
+    f3 origin, dir;
+    camera_ray32(W.cam, x, y, W.inv_w, W.inv_h, rng, &origin, &dir);
     W.ps.o[i] = make_float4(origin.x, origin.y, origin.z, 0.f);
     W.ps.d[i] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
     W.ps.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
@@ -254,24 +121,9 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shade(WfParams W, int cur) {
         }
         if (go) {
           const float t = __int_as_float(h4.x);
-          f3 p = o + d * t;
-          f3 ng;
+          f3 p, ng;
           int mat;
-          if ((uint32_t)h4.y & TPT_ITEM_SPHERE) {
-            int s = (int)((uint32_t)h4.y & ~TPT_ITEM_SPHERE);
-            const double* sp = W.S.sph + 4 * s;
-            ng = (p - mk((float)sp[0], (float)sp[1], (float)sp[2])) / (float)sp[3];
-            mat = W.S.sph_mat[s];
-          } else {
-            const InstF32& I = W.S.inst[h4.y];
-            const float4* tp = reinterpret_cast<const float4*>(W.S.tris + h4.z);
-            f3 no = cross(f3of(tp[1]), f3of(tp[2]));
-            const float4* nm = reinterpret_cast<const float4*>(I.nrm);
-            float4 n0 = nm[0], n1 = nm[1], n2 = nm[2];
-            ng = vnorm(mk(n0.x * no.x + n0.y * no.y + n0.z * no.z, n1.x * no.x + n1.y * no.y + n1.z * no.z,
-                          n2.x * no.x + n2.y * no.y + n2.z * no.z));
-            mat = I.mat_override >= 0 ? I.mat_override : W.S.tri_mat[h4.z];
-          }
+          hit_geometry32(W.S, h4.y, h4.z, t, o, d, &p, &ng, &mat);
           const int kind = W.mats.kind[mat];
           const float4 mp = W.mats.p32[mat];
           const f3 mc = f3of(mp);
@@ -312,9 +164,10 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shade(WfParams W, int cur) {
                     float pdf_b;
                     bsdf_eval(kind, mc, mp.w, wo, wi, n, &f, &pdf_b);
                     f3 c = had(had(beta, f), ls.emit) * (cs * mis_weight(ls.pdf, pdf_b) / ls.pdf);
-                    if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
+                    if ((c.x != 0.f || c.y != 0.f || c.z != 0.f) &&
+                        !light_self_occluded64(W.S64, W.lights, ls.idx, p.x, p.y, p.z, ls.u, ls.v)) {
                       f3 so = p + (float)TPT_RAY_EPSILON * wi;
-                      float tmax = fmaxf(dist - (float)TPT_RAY_EPSILON, (float)TPT_RAY_EPSILON);
+                      float tmax = light_shadow_tmax32(dist);
                       W.ps.sh_o[2 * slot + 1] = make_float4(so.x, so.y, so.z, tmax);
                       W.ps.sh_d[2 * slot + 1] = make_float4(wi.x, wi.y, wi.z, 0.f);
                       sh_c1 = make_float4(c.x, c.y, c.z, 1.f);

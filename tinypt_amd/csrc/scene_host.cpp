@@ -547,9 +547,10 @@ void HostScene::rebuild_lights() {
   double total = 0.0;
   auto emits = [&](int mid) { return mid >= 0 && mid < (int)mat_kind.size() && mat_kind[(size_t)mid] == TPT_MAT_DIFFUSE_LIGHT; };
   auto add = [&](LightF64 L, double area, const double* world_v) {
-    const double* e = &mat_param[4 * (size_t)(L.kind == 0 ? sph_mat[(size_t)L.idx]
-                                                          : (inst_override[(size_t)L.inst] >= 0 ? inst_override[(size_t)L.inst]
-                                                                                                : tri_mat[(size_t)(mesh_tri_begin[(size_t)L.mesh] + L.tri)]))];
+    const int mid = L.kind == 0 ? sph_mat[(size_t)L.idx]
+                                : (inst_override[(size_t)L.inst] >= 0 ? inst_override[(size_t)L.inst]
+                                                                      : tri_mat[(size_t)(mesh_tri_begin[(size_t)L.mesh] + L.tri)]);
+    const double* e = &mat_param[4 * (size_t)mid];
     double weight = area * lum(e);
     if (!(weight > 0.0)) return;
     total += weight;
@@ -559,6 +560,7 @@ void HostScene::rebuild_lights() {
     light_cdf.push_back(total);
     LightF32 f{};
     f.kind = L.kind;
+    f.mat = mid;
     if (L.kind == 0) {
       for (int k = 0; k < 4; k++) f.p0[k] = (float)sph[4 * (size_t)L.idx + (size_t)k];
     } else {

@@ -53,7 +53,8 @@ struct alignas(16) LightF32 {
   float p2[4];  // triangle: v2 world
   float emit[4];  // rgb, selection weight
   int32_t kind;   // 0 sphere, 1 triangle
-  int32_t pad[3];
+  int32_t mat;    // emitting material
+  int32_t pad[2];
 };
 
 // ---- FP64, reference layout

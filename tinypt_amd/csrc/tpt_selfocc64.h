@@ -1,0 +1,118 @@
+// tpt_selfocc64.h -- the reference's shadow test of a sampled area-light point against the
+// light's own surface, restated in FP64 for the FP32 kernels.
+//
+// nee_area_light (integrator.rs:178-213) shoots the shadow ray from hit_p + 1e-4 wi with
+// tmax = dist - 1e-4, where dist = |pos - hit_p|: the light surface itself lies at t = tmax up to
+// rounding.  World::hit accepts an instance hit when t_world < tmax (world.rs:104) and a sphere
+// hit when t <= tmax (geometry.rs:324-329), so whether the light shadows its own sample is
+// decided by the last bits of the f64 computation -- about 30% of samples are self-occluded on
+// a rectangle light.  FP32 rounding gives a very different rate.  The FP32 kernels therefore
+// trace their shadow ray with the light's own surface excluded (tmax pulled in by 1e-5
+// relative) and take the self-occlusion decision from this function, which repeats the
+// reference's f64 operations in the reference's order on the FP32 hit point and uniforms.
+// Contraction is off inside the function so each operation rounds as the reference's does.
+#pragma once
+
+#include "tpt_kernels.h"
+
+namespace tpt {
+
+struct D3 {
+  double x, y, z;
+};
+
+// Light index `idx`, shading point p (FP32 promoted), light-sample uniforms (u, v).
+__device__ __noinline__ bool light_self_occluded64(const SceneF64& S, const LightsDev& Ls, int idx, float px, float py,
+                                                   float pz, float uf, float vf) {
+#pragma clang fp contract(off)
+  auto add = [](D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; };
+  auto sub = [](D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; };
+  auto mul = [](D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; };
+  auto dot = [](D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+  auto cross = [](D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+  auto mv = [](const double* m, D3 v) {
+    return D3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z};
+  };
+  auto ld = [](const double* p) { return D3{p[0], p[1], p[2]}; };
+  const double eps = 1e-4;
+  const LightF64& L = Ls.l64[idx];
+  const D3 hit_p{(double)px, (double)py, (double)pz};
+  const double u = (double)uf, v = (double)vf;
+  if (L.kind == 0) {
+    // Light::sample_surface for a sphere (world.rs:244-255), then Sphere::hit (geometry.rs:174-195)
+    const double* s = S.sph + 4 * L.idx;
+    const D3 c = ld(s);
+    const double rs = s[3];
+    double z = 1.0 - 2.0 * u;
+    double r = sqrt(fmax(1.0 - z * z, 0.0));
+    double phi = 6.28318530717958647692 * v;
+    D3 dir{r * cos(phi), z, r * sin(phi)};
+    D3 pos = add(c, mul(dir, rs));
+    D3 tl = sub(pos, hit_p);
+    double dist2 = dot(tl, tl);
+    double dist = sqrt(dist2);
+    D3 wi = D3{tl.x / dist, tl.y / dist, tl.z / dist};
+    D3 o = add(hit_p, mul(wi, eps));
+    double tmax = fmax(dist - eps, eps);
+    D3 oc = sub(o, c);
+    double a = dot(wi, wi);
+    double b = dot(oc, wi);
+    double cc = dot(oc, oc) - rs * rs;
+    double disc = b * b - a * cc;
+    if (disc <= 0.0) return false;
+    double sd = sqrt(disc);
+    double t = (-b - sd) / a;
+    if (t < eps || t > tmax) {
+      t = (-b + sd) / a;
+      if (t < eps || t > tmax) return false;
+    }
+    return true;
+  }
+  // triangle light: tri_world_verts (world.rs:276-292), sample point (world.rs:256-269)
+  const InstF64& I = S.inst[L.inst];
+  const double* tv = S.tris + 9 * (size_t)(S.meshes[L.mesh].tri_off + L.tri);
+  const D3 v0 = ld(tv), v1 = ld(tv + 3), v2 = ld(tv + 6);
+  const D3 t = ld(I.t);
+  const D3 w0 = add(mv(I.a, v0), t), w1 = add(mv(I.a, v1), t), w2 = add(mv(I.a, v2), t);
+  double su = sqrt(u);
+  double b0 = 1.0 - su, b1 = v * su, b2 = 1.0 - b0 - b1;
+  D3 pos = add(add(mul(w0, b0), mul(w1, b1)), mul(w2, b2));
+  D3 tl = sub(pos, hit_p);
+  double dist2 = dot(tl, tl);
+  double dist = sqrt(dist2);
+  D3 wi = D3{tl.x / dist, tl.y / dist, tl.z / dist};
+  D3 o = add(hit_p, mul(wi, eps));
+  double tmax = fmax(dist - eps, eps);
+  // World::hit for this instance (world.rs:92-108), Triangle::hit on the sampled triangle
+  D3 o_obj = mv(I.a_inv, sub(o, t));
+  D3 d_obj = mv(I.a_inv, wi);
+  double l = fmax(sqrt(dot(d_obj, d_obj)), 1e-30);
+  d_obj = D3{d_obj.x / l, d_obj.y / l, d_obj.z / l};
+  D3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+  D3 pvec = cross(d_obj, e2);
+  double det = dot(e1, pvec);
+  if (fabs(det) < 1e-10) return false;
+  double inv_det = 1.0 / det;
+  D3 tvec = sub(o_obj, v0);
+  double uu = dot(tvec, pvec) * inv_det;
+  if (uu < 0.0 || uu > 1.0) return false;
+  D3 qvec = cross(tvec, e1);
+  double vv = dot(d_obj, qvec) * inv_det;
+  if (vv < 0.0 || uu + vv > 1.0) return false;
+  double th = dot(e2, qvec) * inv_det;
+  if (th < eps || th > 1e30) return false;
+  D3 p_obj = add(o_obj, mul(d_obj, th));
+  D3 p_world = add(mv(I.a, p_obj), t);
+  double t_world = dot(sub(p_world, o), wi);
+  return t_world > eps && t_world < tmax;
+}
+
+// Far end of the FP32 shadow ray towards a sampled light point at distance `dist`: the
+// reference's tmax = max(dist - 1e-4, 1e-4) pulled in by 2e-4 relative (at least 4e-6), which
+// keeps the FP32 ray off the light's own surface for all but grazing (cos < ~1e-3) arrivals.
+__device__ __forceinline__ float light_shadow_tmax32(float dist) {
+  const float eps = 1e-4f;
+  return fmaxf(dist - eps, eps) - fmaxf(2e-4f * dist, 4e-6f);
+}
+
+}  // namespace tpt

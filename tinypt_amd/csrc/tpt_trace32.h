@@ -90,13 +90,16 @@ TPT_HD bool sphere_hit64(const double* s, vec3<float> of, vec3<float> df, double
 }
 
 // Returns true on a hit.  ANY: stop at the first hit in range (shadow rays).
+// Loop shape: an inner loop descends through inner nodes (near child kept in a register, far
+// child pushed); the outer loop then handles the leaf, top-level item or instance exit that
+// the descent stopped at, and pops the next entry.
 template <bool ANY>
 TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmin, float tmax, HitF32* hit,
-                                          const LaneStack& st) {
-  int32_t ref = S.tlas_root;
+                      const LaneStack& st) {
   hit->t = tmax;
   hit->inst = -1;
   hit->prim = -1;
+  int32_t ref = S.tlas_root;
   if (ref == TPT_REF_EMPTY) return false;
   const vec3<float> inv_w = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
   const vec3<float> oi_w = mk(o.x * inv_w.x, o.y * inv_w.y, o.z * inv_w.z);
@@ -110,96 +113,105 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
   int sp = 0;
   bool found = false;
   for (;;) {
-    if (ref >= 0 && ref != TPT_STACK_SENTINEL) {
-      // ---- inner node: both children's boxes from one 64 B node
+    // ---- descend inner nodes: both children's boxes from one 64 B node
+    bool have = true;
+    while (ref >= 0 && ref != TPT_STACK_SENTINEL) {
       const float4* p = reinterpret_cast<const float4*>(S.nodes + ref);
       const float4 a = p[0], b = p[1], c = p[2];
       const int4 m = reinterpret_cast<const int4*>(p)[3];
       float x0 = fmaf(a.x, inv.x, -oi.x), x1 = fmaf(a.w, inv.x, -oi.x);
       float y0 = fmaf(a.y, inv.y, -oi.y), y1 = fmaf(b.x, inv.y, -oi.y);
       float z0 = fmaf(a.z, inv.z, -oi.z), z1 = fmaf(b.y, inv.z, -oi.z);
-      float ln = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
-      float lf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
+      const float ln = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
+      const float lf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
       x0 = fmaf(b.z, inv.x, -oi.x); x1 = fmaf(c.y, inv.x, -oi.x);
       y0 = fmaf(b.w, inv.y, -oi.y); y1 = fmaf(c.z, inv.y, -oi.y);
       z0 = fmaf(c.x, inv.z, -oi.z); z1 = fmaf(c.w, inv.z, -oi.z);
-      float rn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
-      float rf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
-      bool hl = ln <= lf, hr = rn <= rf;
+      const float rn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
+      const float rf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
+      const bool hl = ln <= lf, hr = rn <= rf;
       if (hl && hr) {
         // nearer first; the reference pushes far then near with `a_t0 <= b_t0` choosing left
-        bool left_first = ln <= rn;
-        ref = left_first ? m.x : m.y;
+        const bool left_first = ln <= rn;
         st.push(sp, left_first ? m.y : m.x);
-        continue;
+        ref = left_first ? m.x : m.y;
+      } else if (hl) {
+        ref = m.x;
+      } else if (hr) {
+        ref = m.y;
+      } else {
+        have = false;
+        break;
       }
-      if (hl) { ref = m.x; continue; }
-      if (hr) { ref = m.y; continue; }
-    } else if (ref == TPT_STACK_SENTINEL) {
-      // ---- leaving an instance: back to the world ray
-      co = o; cd = d; inv = inv_w; oi = oi_w; lo = tmin; det_eps = 1e-10f; cur_inst = -1;
-    } else if (cur_inst < 0) {
-      // ---- top-level leaf: one item
-      int32_t item = S.tlas_items[(uint32_t)ref & TPT_LEAF_START_MASK];
-      if ((uint32_t)item & TPT_ITEM_SPHERE) {
-        int32_t s = (int32_t)((uint32_t)item & ~TPT_ITEM_SPHERE);
-        double th;
-        if (sphere_hit64(S.sph + 4 * s, o, d, (double)tmin, (double)best_t, &th)) {
-          float tf = (float)th;
-          int32_t key = TPT_KEY_SPHERE(s);
-          if (tf < best_t || key >= best_key) {
-            best_t = tf; best_key = key; found = true;
-            hit->t = tf; hit->inst = (int32_t)((uint32_t)s | TPT_ITEM_SPHERE); hit->prim = -1;
-            if (ANY) return true;
+    }
+    if (have) {
+      if (ref == TPT_STACK_SENTINEL) {
+        // ---- leaving an instance: back to the world ray
+        co = o; cd = d; inv = inv_w; oi = oi_w; lo = tmin; det_eps = 1e-10f; cur_inst = -1;
+      } else if (cur_inst < 0) {
+        // ---- top-level leaf: one item
+        const int32_t item = S.tlas_items[(uint32_t)ref & TPT_LEAF_START_MASK];
+        if ((uint32_t)item & TPT_ITEM_SPHERE) {
+          const int32_t s = (int32_t)((uint32_t)item & ~TPT_ITEM_SPHERE);
+          double th;
+          if (sphere_hit64(S.sph + 4 * s, o, d, (double)tmin, (double)best_t, &th)) {
+            const float tf = (float)th;
+            const int32_t key = TPT_KEY_SPHERE(s);
+            if (tf < best_t || key >= best_key) {
+              best_t = tf; best_key = key; found = true;
+              hit->t = tf; hit->inst = (int32_t)((uint32_t)s | TPT_ITEM_SPHERE); hit->prim = -1;
+              if (ANY) return true;
+            }
+          }
+        } else {
+          const InstF32& I = S.inst[item];
+          const int32_t root = I.root;
+          if (root != TPT_REF_EMPTY) {
+            const float4* w = reinterpret_cast<const float4*>(I.w2o);
+            const float4 r0 = w[0], r1 = w[1], r2 = w[2];
+            co = mk(fmaf(r0.x, o.x, fmaf(r0.y, o.y, fmaf(r0.z, o.z, r0.w))),
+                    fmaf(r1.x, o.x, fmaf(r1.y, o.y, fmaf(r1.z, o.z, r1.w))),
+                    fmaf(r2.x, o.x, fmaf(r2.y, o.y, fmaf(r2.z, o.z, r2.w))));
+            cd = mk(fmaf(r0.x, d.x, fmaf(r0.y, d.y, r0.z * d.z)), fmaf(r1.x, d.x, fmaf(r1.y, d.y, r1.z * d.z)),
+                    fmaf(r2.x, d.x, fmaf(r2.y, d.y, r2.z * d.z)));
+            const float len = sqrtf(dot(cd, cd));
+            lo = fmaxf(tmin, tmin / len);
+            det_eps = 1e-10f * len;
+            inv = mk(safe_rcp(cd.x), safe_rcp(cd.y), safe_rcp(cd.z));
+            oi = mk(co.x * inv.x, co.y * inv.y, co.z * inv.z);
+            cur_inst = item;
+            cur_key = TPT_KEY_INST(item);
+            st.push(sp, TPT_STACK_SENTINEL);
+            ref = root;
+            continue;
           }
         }
       } else {
-        const InstF32& I = S.inst[item];
-        const float4* w = reinterpret_cast<const float4*>(I.w2o);
-        const float4 r0 = w[0], r1 = w[1], r2 = w[2];
-        int32_t root = I.root;
-        if (root != TPT_REF_EMPTY) {
-          co = mk(fmaf(r0.x, o.x, fmaf(r0.y, o.y, fmaf(r0.z, o.z, r0.w))), fmaf(r1.x, o.x, fmaf(r1.y, o.y, fmaf(r1.z, o.z, r1.w))),
-                  fmaf(r2.x, o.x, fmaf(r2.y, o.y, fmaf(r2.z, o.z, r2.w))));
-          cd = mk(fmaf(r0.x, d.x, fmaf(r0.y, d.y, r0.z * d.z)), fmaf(r1.x, d.x, fmaf(r1.y, d.y, r1.z * d.z)),
-                  fmaf(r2.x, d.x, fmaf(r2.y, d.y, r2.z * d.z)));
-          float len = sqrtf(dot(cd, cd));
-          lo = fmaxf(tmin, tmin / len);
-          det_eps = 1e-10f * len;
-          inv = mk(safe_rcp(cd.x), safe_rcp(cd.y), safe_rcp(cd.z));
-          oi = mk(co.x * inv.x, co.y * inv.y, co.z * inv.z);
-          cur_inst = item;
-          cur_key = TPT_KEY_INST(item);
-          st.push(sp, TPT_STACK_SENTINEL);
-          ref = root;
-          continue;
-        }
-      }
-    } else {
-      // ---- bottom-level leaf: up to 8 triangles in consecutive slots
-      const uint32_t start = (uint32_t)ref & TPT_LEAF_START_MASK;
-      const uint32_t cnt = (((uint32_t)ref >> 28) & 7u) + 1u;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const float4* tp = reinterpret_cast<const float4*>(S.tris + start + k);
-        const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
-        // Moller-Trumbore (geometry.rs:236-273)
-        vec3<float> E1 = mk(e1.x, e1.y, e1.z), E2 = mk(e2.x, e2.y, e2.z);
-        vec3<float> pvec = cross(cd, E2);
-        float det = dot(E1, pvec);
-        if (fabsf(det) < det_eps) continue;
-        float inv_det = 1.0f / det;
-        vec3<float> tvec = mk(co.x - v0.x, co.y - v0.y, co.z - v0.z);
-        float u = dot(tvec, pvec) * inv_det;
-        if (u < 0.f || u > 1.f) continue;
-        vec3<float> qvec = cross(tvec, E1);
-        float v = dot(cd, qvec) * inv_det;
-        if (v < 0.f || u + v > 1.f) continue;
-        float th = dot(E2, qvec) * inv_det;
-        if (th < lo || th > best_t) continue;
-        if (th < best_t || cur_key >= best_key) {
-          best_t = th; best_key = cur_key; found = true;
-          hit->t = th; hit->inst = cur_inst; hit->prim = (int32_t)(start + k);
-          if (ANY) return true;
+        // ---- bottom-level leaf: up to 8 triangles in consecutive slots
+        const uint32_t start = (uint32_t)ref & TPT_LEAF_START_MASK;
+        const uint32_t cnt = (((uint32_t)ref >> 28) & 7u) + 1u;
+        for (uint32_t k = 0; k < cnt; k++) {
+          const float4* tp = reinterpret_cast<const float4*>(S.tris + start + k);
+          const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
+          // Moller-Trumbore (geometry.rs:236-273)
+          const vec3<float> E1 = mk(e1.x, e1.y, e1.z), E2 = mk(e2.x, e2.y, e2.z);
+          const vec3<float> pvec = cross(cd, E2);
+          const float det = dot(E1, pvec);
+          if (fabsf(det) < det_eps) continue;
+          const float inv_det = 1.0f / det;
+          const vec3<float> tvec = mk(co.x - v0.x, co.y - v0.y, co.z - v0.z);
+          const float u = dot(tvec, pvec) * inv_det;
+          if (u < 0.f || u > 1.f) continue;
+          const vec3<float> qvec = cross(tvec, E1);
+          const float v = dot(cd, qvec) * inv_det;
+          if (v < 0.f || u + v > 1.f) continue;
+          const float th = dot(E2, qvec) * inv_det;
+          if (th < lo || th > best_t) continue;
+          if (th < best_t || cur_key >= best_key) {
+            best_t = th; best_key = cur_key; found = true;
+            hit->t = th; hit->inst = cur_inst; hit->prim = (int32_t)(start + k);
+            if (ANY) return true;
+          }
         }
       }
     }

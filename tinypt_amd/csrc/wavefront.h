@@ -10,7 +10,8 @@
 namespace tpt {
 
 #define TPT_WF_BLOCK 256
-#define TPT_WF_STACK 24  // LDS stack entries per lane (24 KB per 256-lane block)
+#define TPT_WF_STACK 24      // LDS stack entries per lane (24 KB per 256-lane block)
+#define TPT_WF_STACK_ADJ 16  // the adjoint kernel keeps 16 in LDS (plus 4 KB of gradient slots)
 
 #define TPT_CNT_Q0 0
 #define TPT_CNT_Q1 1
@@ -38,6 +39,7 @@ struct PathState {
 
 struct WfParams {
   SceneF32 S;
+  SceneF64 S64;  // reference-layout scene, for the light self-occlusion decision
   EnvDev env;
   LightsDev lights;
   MatsDev mats;
