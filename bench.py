@@ -38,11 +38,12 @@ def log(*a):
 
 
 def algorithmic_bytes_per_ray(counters):
-    """Bytes one World::hit query reads/writes by the reference traversal's work counts: 24 per
-    AABB slab test (FP32 box), 36 per triangle test, 16 per sphere test, 48 per instance
-    visit (FP32 affine inverse), 32 ray in + 16 hit out."""
-    rays = counters["rays_ext"] + counters["rays_shadow"]
-    return (24.0 * counters["aabb"] + 36.0 * counters["tri"] + 16.0 * counters["sph"] + 48.0 * counters["inst"]) / rays + 48.0
+    """Bytes one extension (closest-hit) World::hit query reads and writes, from the reference
+    traversal's work counts on the same scene: 24 per AABB slab test (FP32 box), 36 per
+    triangle test, 16 per sphere test, 48 per instance visit (FP32 affine inverse), 32 ray in +
+    16 hit out."""
+    c = counters
+    return (24.0 * c["aabb_ext"] + 36.0 * c["tri_ext"] + 16.0 * c["sph_ext"] + 48.0 * c["inst_ext"]) / c["rays_ext"] + 48.0
 
 
 def cpu_baseline(scene, seconds_target=15.0):
@@ -70,6 +71,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-rmse", action="store_true", help="skip the 2048-spp FP64 reference image")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU render")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box here has one GPU).",
@@ -103,15 +107,27 @@ def main():
     img32 = T.resolve_pixels(out.acc, out.acc_w)
     log(f"fp32 wavefront: {secs * 1e3:.1f} ms/step, {value / 1e6:.1f} M paths/s, {mrays:.1f} Mrays/s")
 
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del a, b
+
     # plain FP64 GPU form on the same workload
-    cfg64 = T.RenderConfig(width=W, height=H, spp=SPP, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED,
-                           precision=T.FP64_PLAIN)
-    t0 = time.perf_counter()
-    out64 = T.render(ds, cfg64, want_sq=False)
-    secs64 = time.perf_counter() - t0
-    img64 = T.resolve_pixels(out64.acc, out64.acc_w)
-    plain_rate = paths / secs64
-    log(f"fp64 plain: {secs64 * 1e3:.1f} ms, {plain_rate / 1e6:.1f} M paths/s")
+    plain_rate = None
+    img64 = None
+    if not args.no_plain:
+        cfg64 = T.RenderConfig(width=W, height=H, spp=SPP, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED,
+                               precision=T.FP64_PLAIN)
+        t0 = time.perf_counter()
+        out64 = T.render(ds, cfg64, want_sq=False)
+        secs64 = time.perf_counter() - t0
+        img64 = T.resolve_pixels(out64.acc, out64.acc_w)
+        plain_rate = paths / secs64
+        log(f"fp64 plain: {secs64 * 1e3:.1f} ms, {plain_rate / 1e6:.1f} M paths/s")
 
     rmse = None
     if not args.no_rmse:
@@ -120,7 +136,7 @@ def main():
         ref = T.render(ds, ref_cfg, want_sq=False)
         ref_img = T.resolve_pixels(ref.acc, ref.acc_w)
         rmse = {"fp32_wavefront_256spp": float(np.sqrt(np.mean((img32 - ref_img) ** 2))),
-                "fp64_plain_256spp": float(np.sqrt(np.mean((img64 - ref_img) ** 2))),
+                "fp64_plain_256spp": float(np.sqrt(np.mean((img64 - ref_img) ** 2))) if img64 is not None else None,
                 "reference": "FP64 plain GPU render, 2048 spp, independent seed"}
         log(f"rmse: {rmse}")
 
@@ -149,7 +165,8 @@ def main():
             "config": {"workload": "cornell_box_512x512_256spp_maxdepth12_rrdepth4_seed2", "width": W, "height": H,
                        "spp": SPP, "max_depth": MAX_BOUNCES, "rr_depth": RR_START, "paths_per_step": paths,
                        "accumulation": "fp64"},
-            "mrays_per_s": mrays, "fp64_plain_paths_per_s": plain_rate, "speedup_vs_fp64_plain": value / plain_rate,
+            "mrays_per_s": mrays, "fp64_plain_paths_per_s": plain_rate,
+            "speedup_vs_fp64_plain": value / plain_rate if plain_rate else None,
             "rmse": rmse, "roofline": roofline, "cpu_baseline": cpu}
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]

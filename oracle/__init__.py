@@ -26,7 +26,8 @@ _u64p = C.POINTER(C.c_uint64)
 
 
 class Counters(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("aabb", "tri", "sph", "inst", "rays_ext", "rays_shadow", "paths", "ext_bounces")]
+    _fields_ = [(n, C.c_uint64) for n in ("aabb", "tri", "sph", "inst", "rays_ext", "rays_shadow", "paths", "ext_bounces",
+                                                  "aabb_ext", "tri_ext", "sph_ext", "inst_ext")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

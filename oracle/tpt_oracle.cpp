@@ -168,9 +168,12 @@ struct Transform {
 // ---------------------------------------------------------------- work counters (additions)
 struct Counters {
   uint64_t aabb = 0, tri = 0, sph = 0, inst = 0, rays_ext = 0, rays_shadow = 0, paths = 0, ext_bounces = 0;
+  // the share of aabb / tri / sph / inst spent on extension (closest-hit) rays
+  uint64_t aabb_ext = 0, tri_ext = 0, sph_ext = 0, inst_ext = 0;
   void add(const Counters& o) {
     aabb += o.aabb; tri += o.tri; sph += o.sph; inst += o.inst;
     rays_ext += o.rays_ext; rays_shadow += o.rays_shadow; paths += o.paths; ext_bounces += o.ext_bounces;
+    aabb_ext += o.aabb_ext; tri_ext += o.tri_ext; sph_ext += o.sph_ext; inst_ext += o.inst_ext;
   }
 };
 static thread_local Counters* tl_cnt = nullptr;
@@ -1103,8 +1106,16 @@ static V3 radiance(const Scene& sc, Ray ray, Rng& rng, int max_bounces, int rr_s
   bool last_non_delta = false;
   for (int bounce = 0; bounce < max_bounces; bounce++) {
     Hit hit;
-    if (tl_cnt) { tl_cnt->rays_ext++; tl_cnt->ext_bounces++; }
-    if (!sc.hit(ray, RAY_EPSILON, RAY_T_MAX, &hit)) {
+    Counters before;
+    if (tl_cnt) { tl_cnt->rays_ext++; tl_cnt->ext_bounces++; before = *tl_cnt; }
+    const bool found = sc.hit(ray, RAY_EPSILON, RAY_T_MAX, &hit);
+    if (tl_cnt) {
+      tl_cnt->aabb_ext += tl_cnt->aabb - before.aabb;
+      tl_cnt->tri_ext += tl_cnt->tri - before.tri;
+      tl_cnt->sph_ext += tl_cnt->sph - before.sph;
+      tl_cnt->inst_ext += tl_cnt->inst - before.inst;
+    }
+    if (!found) {
       V3 bg = sc.has_env ? sc.env.sample(ray.d) : sky(ray.d);
       V3 contrib = had(beta, bg);
       if (sc.has_env && last_non_delta) {
@@ -1329,6 +1340,7 @@ extern "C" {
 
 typedef struct orc_counters {
   uint64_t aabb, tri, sph, inst, rays_ext, rays_shadow, paths, ext_bounces;
+  uint64_t aabb_ext, tri_ext, sph_ext, inst_ext;  // part of the above spent on extension rays
 } orc_counters;
 
 typedef struct orc_render_opts {
@@ -1510,7 +1522,9 @@ void orc_scene_trace(void* s, int64_t n, const double* rays, double* out, orc_co
   if (cnt) {
     Counters tot;
     for (auto& c : cs) tot.add(c);
-    *cnt = orc_counters{tot.aabb, tot.tri, tot.sph, tot.inst, tot.rays_ext, tot.rays_shadow, tot.paths, tot.ext_bounces};
+    // every query here is a closest-hit query
+    *cnt = orc_counters{tot.aabb, tot.tri, tot.sph, tot.inst, tot.rays_ext, tot.rays_shadow, tot.paths, tot.ext_bounces,
+                        tot.aabb, tot.tri, tot.sph, tot.inst};
   }
 }
 // env (env.rs): only for TPT_ENV_IMAGE scenes
@@ -1561,7 +1575,8 @@ double orc_render(void* s, const orc_render_opts* ro, double* acc, double* acc_w
   auto t0 = std::chrono::steady_clock::now();
   render(*sc, o, acc, acc_w, acc_sq, &tot);
   auto t1 = std::chrono::steady_clock::now();
-  if (cnt) *cnt = orc_counters{tot.aabb, tot.tri, tot.sph, tot.inst, tot.rays_ext, tot.rays_shadow, tot.paths, tot.ext_bounces};
+  if (cnt) *cnt = orc_counters{tot.aabb, tot.tri, tot.sph, tot.inst, tot.rays_ext, tot.rays_shadow, tot.paths, tot.ext_bounces,
+                        tot.aabb_ext, tot.tri_ext, tot.sph_ext, tot.inst_ext};
   return std::chrono::duration<double>(t1 - t0).count();
 }
 // output.rs:96-102

@@ -27,7 +27,7 @@ from .scene import (Camera, DiffuseLight, Dielectric, EnvMap, Ggx, Instance, Lam
 __all__ = ["Camera", "DiffuseLight", "Dielectric", "EnvMap", "Ggx", "Instance", "Lambert", "Material", "Mesh", "Metal",
            "RenderConfig", "Scene", "Sphere", "Subsurface", "Transform", "World", "Context", "DeviceScene", "HostScene",
            "RenderOutput",
-           "render", "render_adjoint", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
+           "render", "render_device", "render_adjoint", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
 
 import atexit
 import weakref
@@ -252,6 +252,26 @@ def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Cont
     _check(lib.tpt_render(dscene.ctx.h, dscene.h, C.byref(cfg), _dp(acc), _dp(acc_w),
                           _dp(acc_sq) if want_sq else None, C.byref(st)))
     return RenderOutput(acc, acc_w, acc_sq, st.as_dict())
+
+
+def render_device(scene: "DeviceScene", config: RenderConfig, acc, acc_w, acc_sq=None) -> dict:
+    """``tpt_render_device``: render into caller-owned device buffers (e.g. float64 torch
+    tensors on the context's device, acc (h, w, 3), acc_w (h, w), acc_sq (h, w, 3) or None).
+    Synchronous with respect to the host.  Returns the run counters."""
+    h, w = int(config.height), int(config.width)
+    for name, t, n in (("acc", acc, h * w * 3), ("acc_w", acc_w, h * w), ("acc_sq", acc_sq, h * w * 3)):
+        if t is None:
+            continue
+        if getattr(t, "dtype", None) is not None and str(t.dtype) not in ("torch.float64", "float64"):
+            raise ValueError(f"{name} must be float64")
+        if hasattr(t, "numel") and (t.numel() != n or not t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous buffer of {n} doubles")
+    ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    st = _abi.Stats()
+    cfg = config.to_c()
+    _check(scene._lib.tpt_render_device(scene.ctx.h, scene.h, C.byref(cfg), ptr(acc), ptr(acc_w), ptr(acc_sq),
+                                        C.byref(st)))
+    return st.as_dict()
 
 
 def render_adjoint(scene: "DeviceScene", config: RenderConfig, d_loss_d_pixel: np.ndarray,

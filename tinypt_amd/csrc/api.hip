@@ -375,8 +375,13 @@ int check_cfg(const tpt_render_config* c) {
   return TPT_OK;
 }
 
+// Deepest traversal stack any ray can need: one entry per TLAS level, the instance sentinel,
+// one entry per BLAS level, plus one of slack.
+int stack_need(const HostScene& H) { return H.max_tlas_depth + H.max_blas_depth + 2; }
+
+// Global overflow entries per lane beyond the TPT_WF_STACK entries kept in LDS.
 int needed_gstack(const HostScene& H) {
-  int need = H.max_tlas_depth + H.max_blas_depth + 2;
+  int need = stack_need(H);
   return need > TPT_WF_STACK ? need - TPT_WF_STACK : 0;
 }
 
@@ -722,7 +727,7 @@ int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_co
   if (rc) return rc;
   HIPC(hipSetDevice(ctx->device));
   return adjoint_render(ctx->stream, scene->s32, scene->s64, scene->env, scene->lights, scene->mats, scene->cam32,
-                        needed_gstack(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
+                        stack_need(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
 }
 
 }  // extern "C"

@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(TPT_ADJ_BLOCK) k_adjoint(AdjParams A) {
 }
 
 int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const EnvDev& env, const LightsDev& lights, const MatsDev& mats,
-                   const CamF32& cam, int gstack_need, const tpt_render_config* cfg, const float* d_loss_d_pixel,
+                   const CamF32& cam, int stack_need, const tpt_render_config* cfg, const float* d_loss_d_pixel,
                    double* grad_params, tpt_stats* stats, std::string* err) {
   auto fail = [&](int code, const std::string& m) {
     *err = m;
@@ -348,7 +348,7 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   const size_t npix = (size_t)cfg->width * cfg->height;
   const uint64_t n_paths = (uint64_t)npix * (uint64_t)std::max(cfg->spp, 0);
   const int blocks = (int)std::min<uint64_t>((n_paths + TPT_ADJ_BLOCK - 1) / TPT_ADJ_BLOCK, 8192);
-  const int gcap = gstack_need > 0 ? gstack_need + TPT_WF_STACK - TPT_WF_STACK_ADJ : 0;
+  const int gcap = stack_need > TPT_WF_STACK_ADJ ? stack_need - TPT_WF_STACK_ADJ : 0;
   void *d_dldp = nullptr, *d_grad = nullptr, *d_stats = nullptr, *d_gs = nullptr;
   auto cleanup = [&]() {
     if (d_dldp) (void)hipFree(d_dldp);
