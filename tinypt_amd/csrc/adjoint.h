@@ -12,7 +12,7 @@
 namespace tpt {
 
 int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const EnvDev& env, const LightsDev& lights, const MatsDev& mats,
-                   const CamF32& cam, int stack_need, const tpt_render_config* cfg, const float* d_loss_d_pixel,
-                   double* grad_params, tpt_stats* stats, std::string* err);
+                   const CamF32& cam, const double* mat_param, int stack_need, const tpt_render_config* cfg,
+                   const float* d_loss_d_pixel, double* grad_params, tpt_stats* stats, std::string* err);
 
 }  // namespace tpt

@@ -727,7 +727,7 @@ int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_co
   if (rc) return rc;
   HIPC(hipSetDevice(ctx->device));
   return adjoint_render(ctx->stream, scene->s32, scene->s64, scene->env, scene->lights, scene->mats, scene->cam32,
-                        stack_need(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
+                        scene->host.mat_param.data(), stack_need(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
 }
 
 }  // extern "C"

@@ -12,11 +12,21 @@
 //     weight w_j scales every later term, so its parameters receive
 //     sum_c (dw_jc/dtheta / w_jc) * (S_total - S_upto_j)_c, the suffix of the pass-A sum.
 // Sampling pdfs (BSDF, light selection, env) and Russian-roulette probabilities are held fixed:
-// the estimator divides by them, so differentiating the integrand alone gives the derivative
-// of the expected image.  Parameters: per material 4 slots laid out like mat_param -- Lambert /
-// Metal / Subsurface albedo rgb, GGX F0 rgb + alpha, Dielectric absorption rgb, DiffuseLight
+// none of these parameters moves a sample, so differentiating the integrand alone gives the
+// derivative of the expected image.  Parameters: per material 4 slots laid out like mat_param --
+// Lambert / Metal / Subsurface albedo rgb, GGX F0 rgb, Dielectric absorption rgb, DiffuseLight
 // emit rgb.
+//
+// GGX roughness is the exception.  The VNDF sampler of material.rs:327-347 does not draw
+// directions with the density its pdf formula (material.rs:283-293) states, and the support
+// of the density it does draw from has an edge that moves with alpha, so the expected image
+// depends on alpha through the sampler in a way no per-sample derivative captures.  Its
+// gradient is therefore taken by replay with common random numbers: every path that meets a
+// GGX material is replayed with that material's alpha at +-TPT_ADJ_ALPHA_H and the central
+// difference of its loss contribution is added.  The sum over paths is the central difference
+// of the whole render's loss with the same seeds, computed in one pass.
 #include <string>
+#include <vector>
 
 #include "adjoint.h"
 #include "tpt_sample32.h"
@@ -27,6 +37,25 @@ namespace tpt {
 
 #define TPT_ADJ_BLOCK 256
 #define TPT_ADJ_LDS_GRAD 1024  // per-block gradient slots in LDS (256 materials)
+#define TPT_ADJ_ALPHA_H 1e-3  // roughness step of the per-path central difference
+#define TPT_ADJ_MAX_GGX 16     // distinct GGX materials per path that receive a roughness gradient
+
+// Replay modes: SUM returns S_total (and lists the path's GGX materials), GRAD accumulates the
+// gradients, VALUE returns the path's loss contribution delta . L with one material's roughness
+// shifted.
+#define ADJ_SUM 0
+#define ADJ_GRAD 1
+#define ADJ_VALUE 2
+
+struct GgxList {
+  int n;
+  int mat[TPT_ADJ_MAX_GGX];
+  __device__ __forceinline__ void add(int m) {
+    for (int i = 0; i < n; i++)
+      if (mat[i] == m) return;
+    if (n < TPT_ADJ_MAX_GGX) mat[n++] = m;
+  }
+};
 
 struct AdjParams {
   SceneF32 S;
@@ -42,6 +71,7 @@ struct AdjParams {
   uint64_t seed;
   uint64_t n_paths;
   const float* dldp;  // w*h*3
+  const float* alpha_pm;  // n_mats*2: (float)(alpha + h), (float)(alpha - h)
   double* grad;       // n_mats*4
   uint64_t* stats;    // [0] extension rays, [1] shadow rays, [2] paths
 };
@@ -56,53 +86,19 @@ __device__ __forceinline__ f3 clamp_vjp(f3 x, f3 v) {
   return (m / l) * v - (m * vx / (l * l)) * lam;
 }
 
-// GGX helpers: derivatives with respect to alpha (material.rs:272-293).
-__device__ __forceinline__ float d_ggx_distribution_da(float a, float c) {
-  if (c <= 0.f) return 0.f;
-  float a2 = a * a;
-  float k = c * c * (a2 - 1.f) + 1.f;
-  return 2.f * a * (k - 2.f * a2 * c * c) / ((float)TPT_PI * k * k * k);
-}
-__device__ __forceinline__ float d_ggx_g1_da(float a, float c) {
-  if (c <= 0.f) return 0.f;
-  float t = sqrtf(fmaxf(1.f - c * c, 0.f)) / fmaxf(c, 1e-6f);
-  float x = a * t;
-  float r = sqrtf(1.f + x * x);
-  return -2.f * (x / r) / ((1.f + r) * (1.f + r)) * t;
-}
-
-// d/d(F0, alpha) of ggx_eval and ggx_pdf at (n, wo, wi).  f_dF0: per channel dF_c/dF0_c * D G /
-// denom; f_da: dF/d alpha per channel; pdf_da: d pdf / d alpha.
-struct GgxGrad {
-  f3 f_dF0;
-  f3 f_da;
-  float pdf_da;
-};
-__device__ __forceinline__ GgxGrad ggx_grad(f3 albedo, float s, f3 n, f3 wo, f3 wi) {
-  GgxGrad g{mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), 0.f};
+// d ggx_eval / dF0_c at (n, wo, wi) (material.rs:272-281): (1 - (1 - |wi.m|)^5) D G / denom,
+// the same for every channel.
+__device__ __forceinline__ float ggx_df_dF0(float s, f3 n, f3 wo, f3 wi) {
   float alpha = fmaxf(s, 1e-3f);
-  float da_ds = s > 1e-3f ? 1.f : 0.f;
   float ci = dot(n, wo), co = dot(n, wi);
-  if (ci <= 0.f || co <= 0.f) return g;
+  if (ci <= 0.f || co <= 0.f) return 0.f;
   f3 m = vnorm(wo + wi);
   float ch = fmaxf(dot(n, m), 0.f);
-  float D = ggx_distribution(alpha, ch), dD = d_ggx_distribution_da(alpha, ch);
-  float G1i = ggx_smith_g1(alpha, ci), G1o = ggx_smith_g1(alpha, co);
-  float dG1i = d_ggx_g1_da(alpha, ci), dG1o = d_ggx_g1_da(alpha, co);
-  float G = G1i * G1o, dG = dG1i * G1o + G1i * dG1o;
+  float D = ggx_distribution(alpha, ch);
+  float G = ggx_smith_g1(alpha, ci) * ggx_smith_g1(alpha, co);
   float denom = 4.f * ci * co + 1e-6f;
   float p5 = pow5(1.f - fmaxf(fabsf(dot(wi, m)), 0.f));
-  f3 F = fresnel_schlick(fabsf(dot(wi, m)), albedo);
-  g.f_dF0 = mk(1.f - p5, 1.f - p5, 1.f - p5) * (D * G / denom);
-  g.f_da = F * ((dD * G + D * dG) / denom * da_ds);
-  float pdf_m = D * G1i * ch / fmaxf(ci, 1e-6f);
-  float pden = 4.f * fmaxf(fabsf(dot(wi, m)), 1e-6f);
-  if (pdf_m / pden > 0.f) g.pdf_da = (dD * G1i + D * dG1i) * ch / fmaxf(ci, 1e-6f) / pden * da_ds;
-  return g;
-}
-__device__ __forceinline__ float dmis_dpb(float pa, float pb) {  // d/dpb of pa^2 / (pa^2 + pb^2)
-  float a2 = pa * pa, b2 = pb * pb, s = a2 + b2;
-  return s > 0.f ? -2.f * a2 * pb / (s * s) : 0.f;
+  return (1.f - p5) * (D * G / denom);
 }
 
 struct GradSink {
@@ -117,17 +113,19 @@ struct GradSink {
   }
 };
 
-// One path.  GRAD = false: returns S_total.  GRAD = true: accumulates gradients given S_total.
-template <bool GRAD>
+// One path (see the modes above).  VALUE: material pert_mat's roughness is replaced by
+// pert_alpha (the FP32 rounding of the FP64 value +- h, as a material update would upload it).
+template <int MODE>
 __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 delta, f3 S_total, GradSink& sink,
-                     const LaneStack& st, uint32_t* n_ext, uint32_t* n_sh) {
+                     GgxList* ggx, int pert_mat, float pert_alpha, const LaneStack& st, uint32_t* n_ext,
+                     uint32_t* n_sh) {
+  constexpr bool GRAD = MODE == ADJ_GRAD;
   RngF32 rng(seed_for(x, y, s) ^ A.seed);
   f3 o, d;
   camera_ray32(A.cam, x, y, A.inv_w, A.inv_h, rng, &o, &d);
   f3 T = mk(1.f, 1.f, 1.f), S_upto = mk(0.f, 0.f, 0.f);
-  float last_pdf = 0.f, last_dpdf = 0.f;
+  float last_pdf = 0.f;
   bool last_nd = false;
-  int last_mat = -1;
   const bool env_img = A.env.kind == TPT_ENV_IMAGE;
   for (int bounce = 0; bounce < A.max_bounces; bounce++) {
     HitF32 h;
@@ -142,10 +140,8 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
         w = mis_weight(last_pdf, pe);
       }
       f3 xw = x0 * w;
-      f3 vj = clamp_vjp(xw, delta);
-      S_upto = S_upto + had(xw, vj);
-      if (GRAD && env_img && last_nd && last_mat >= 0 && A.mats.kind[last_mat] == TPT_MAT_GGX)
-        sink.add(last_mat, 3, -dot(vj, x0) * dmis_dpb(pe, last_pdf) * last_dpdf);  // d/d(first arg)
+      if (MODE == ADJ_VALUE) S_upto = S_upto + had(clamp_luminance(xw, (float)TPT_FIREFLY_CLAMP), delta);
+      else S_upto = S_upto + had(xw, clamp_vjp(xw, delta));
       break;
     }
     if (bounce >= A.rr_start) {  // integrator.rs:89-95 (probability held fixed)
@@ -157,12 +153,14 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
     int mat;
     hit_geometry32(A.S, h.inst, h.prim, h.t, o, d, &p, &ng, &mat);
     const int kind = A.mats.kind[mat];
-    const float4 mp = A.mats.p32[mat];
+    float4 mp = A.mats.p32[mat];
+    if (mat == pert_mat) mp.w = pert_alpha;
     const f3 mc = f3of(mp);
     if (kind == TPT_MAT_DIFFUSE_LIGHT) {  // integrator.rs:100-104
       f3 x0 = had(T, mc);
       f3 vj = clamp_vjp(x0, delta);
-      S_upto = S_upto + had(x0, vj);
+      if (MODE == ADJ_VALUE) S_upto = S_upto + had(clamp_luminance(x0, (float)TPT_FIREFLY_CLAMP), delta);
+      else S_upto = S_upto + had(x0, vj);
       if (GRAD) {
         sink.add(mat, 0, vj.x * T.x);
         sink.add(mat, 1, vj.y * T.y);
@@ -170,6 +168,7 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
       }
       break;
     }
+    if (MODE == ADJ_SUM && kind == TPT_MAT_GGX) ggx->add(mat);
     f3 n = dot(ng, d) < 0.f ? ng : -ng;
     f3 wo = vnorm(-d);
     if (!mat_is_delta(kind)) {
@@ -196,11 +195,10 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
                   sink.add(mat, 1, base.y * w / (float)TPT_PI);
                   sink.add(mat, 2, base.z * w / (float)TPT_PI);
                 } else if (kind == TPT_MAT_GGX) {
-                  GgxGrad g = ggx_grad(mc, mp.w, n, wo, wi);
-                  sink.add(mat, 0, base.x * w * g.f_dF0.x);
-                  sink.add(mat, 1, base.y * w * g.f_dF0.y);
-                  sink.add(mat, 2, base.z * w * g.f_dF0.z);
-                  sink.add(mat, 3, dot(base, g.f_da) * w + dot(base, f) * dmis_dpb(pe, pb) * g.pdf_da);
+                  float g = ggx_df_dF0(mp.w, n, wo, wi) * w;
+                  sink.add(mat, 0, base.x * g);
+                  sink.add(mat, 1, base.y * g);
+                  sink.add(mat, 2, base.z * g);
                 }
               }
             }
@@ -234,11 +232,10 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
                     sink.add(mat, 1, base.y * w / (float)TPT_PI);
                     sink.add(mat, 2, base.z * w / (float)TPT_PI);
                   } else if (kind == TPT_MAT_GGX) {
-                    GgxGrad g = ggx_grad(mc, mp.w, n, wo, wi);
-                    sink.add(mat, 0, base.x * w * g.f_dF0.x);
-                    sink.add(mat, 1, base.y * w * g.f_dF0.y);
-                    sink.add(mat, 2, base.z * w * g.f_dF0.z);
-                    sink.add(mat, 3, dot(base, g.f_da) * w + dot(base, f) * dmis_dpb(ls.pdf, pb) * g.pdf_da);
+                    float g = ggx_df_dF0(mp.w, n, wo, wi) * w;
+                    sink.add(mat, 0, base.x * g);
+                    sink.add(mat, 1, base.y * g);
+                    sink.add(mat, 2, base.z * g);
                   }
                   // emitter radiance of the sampled light
                   f3 e = had(had(T, f), delta) * (cs * w / ls.pdf);
@@ -263,26 +260,14 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
         if (mc.y > 0.f) sink.add(mat, 1, suf.y / mc.y);
         if (mc.z > 0.f) sink.add(mat, 2, suf.z / mc.z);
       } else if (kind == TPT_MAT_GGX) {
-        f3 wi = bs.d;
-        float alpha = fmaxf(mp.w, 1e-3f);
-        float ci = dot(n, wo), co = dot(n, wi);
-        f3 m = vnorm(wo + wi);
-        float ch = fmaxf(dot(n, m), 0.f);
-        float D = ggx_distribution(alpha, ch);
-        float G = ggx_smith(alpha, ci, co);
-        GgxGrad g = ggx_grad(mc, mp.w, n, wo, wi);
-        f3 F = fresnel_schlick(fabsf(dot(wi, m)), mc);
-        float denom = 4.f * ci * co + 1e-6f;
-        float dg = D * G / denom;
-        if (dg > 0.f) {
-          if (F.x > 0.f) sink.add(mat, 0, suf.x * g.f_dF0.x / (F.x * dg));
-          if (F.y > 0.f) sink.add(mat, 1, suf.y * g.f_dF0.y / (F.y * dg));
-          if (F.z > 0.f) sink.add(mat, 2, suf.z * g.f_dF0.z / (F.z * dg));
-          // dlog(D G)/dalpha is the same for all channels
-          float dlog = (g.f_da.x / fmaxf(F.x, 1e-30f)) / dg;
-          sink.add(mat, 3, (suf.x + suf.y + suf.z) * dlog);
-        }
-        last_dpdf = g.pdf_da;
+        // w_c = F_c D G co / (denom pdf): d log w_c / dF0_c = (1 - (1 - |wi.m|)^5) / F_c
+        f3 m = vnorm(wo + bs.d);
+        float c = fabsf(dot(bs.d, m));
+        f3 F = fresnel_schlick(c, mc);
+        float dF = 1.f - pow5(1.f - fmaxf(c, 0.f));
+        if (F.x > 0.f) sink.add(mat, 0, suf.x * dF / F.x);
+        if (F.y > 0.f) sink.add(mat, 1, suf.y * dF / F.y);
+        if (F.z > 0.f) sink.add(mat, 2, suf.z * dF / F.z);
       } else if (kind == TPT_MAT_DIELECTRIC && dot(ng, d) >= 0.f) {
         // exiting: weight carries exp(-sigma_c t) (material.rs:111-119)
         float dist = fmaxf(h.t, 0.f);
@@ -296,7 +281,6 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
     d = bs.d;
     last_pdf = bs.pdf;
     last_nd = !bs.is_delta;
-    last_mat = mat;
   }
   return S_upto;
 }
@@ -317,8 +301,17 @@ __global__ void __launch_bounds__(TPT_ADJ_BLOCK) k_adjoint(AdjParams A) {
     uint32_t x = pix % (uint32_t)A.width, y = pix / (uint32_t)A.width;
     f3 delta = mk(A.dldp[3 * pix], A.dldp[3 * pix + 1], A.dldp[3 * pix + 2]) * (1.f / (float)A.spp);
     if (delta.x == 0.f && delta.y == 0.f && delta.z == 0.f) continue;
-    f3 S_total = replay<false>(A, x, y, s, delta, mk(0.f, 0.f, 0.f), sink, st, &n_ext, &n_sh);
-    replay<true>(A, x, y, s, delta, S_total, sink, st, &n_ext, &n_sh);
+    GgxList ggx;
+    ggx.n = 0;
+    const f3 zero = mk(0.f, 0.f, 0.f);
+    f3 S_total = replay<ADJ_SUM>(A, x, y, s, delta, zero, sink, &ggx, -1, 0.f, st, &n_ext, &n_sh);
+    replay<ADJ_GRAD>(A, x, y, s, delta, S_total, sink, nullptr, -1, 0.f, st, &n_ext, &n_sh);
+    for (int k = 0; k < ggx.n; k++) {  // roughness: central difference of this path's loss term
+      const int m = ggx.mat[k];
+      f3 vp = replay<ADJ_VALUE>(A, x, y, s, delta, zero, sink, nullptr, m, A.alpha_pm[2 * m], st, &n_ext, &n_sh);
+      f3 vm = replay<ADJ_VALUE>(A, x, y, s, delta, zero, sink, nullptr, m, A.alpha_pm[2 * m + 1], st, &n_ext, &n_sh);
+      sink.add(m, 3, ((vp.x - vm.x) + (vp.y - vm.y) + (vp.z - vm.z)) * (float)(0.5 / TPT_ADJ_ALPHA_H));
+    }
     n_paths++;
   }
   __syncthreads();
@@ -338,8 +331,8 @@ __global__ void __launch_bounds__(TPT_ADJ_BLOCK) k_adjoint(AdjParams A) {
 }
 
 int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const EnvDev& env, const LightsDev& lights, const MatsDev& mats,
-                   const CamF32& cam, int stack_need, const tpt_render_config* cfg, const float* d_loss_d_pixel,
-                   double* grad_params, tpt_stats* stats, std::string* err) {
+                   const CamF32& cam, const double* mat_param, int stack_need, const tpt_render_config* cfg,
+                   const float* d_loss_d_pixel, double* grad_params, tpt_stats* stats, std::string* err) {
   auto fail = [&](int code, const std::string& m) {
     *err = m;
     return code;
@@ -349,8 +342,14 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   const uint64_t n_paths = (uint64_t)npix * (uint64_t)std::max(cfg->spp, 0);
   const int blocks = (int)std::min<uint64_t>((n_paths + TPT_ADJ_BLOCK - 1) / TPT_ADJ_BLOCK, 8192);
   const int gcap = stack_need > TPT_WF_STACK_ADJ ? stack_need - TPT_WF_STACK_ADJ : 0;
-  void *d_dldp = nullptr, *d_grad = nullptr, *d_stats = nullptr, *d_gs = nullptr;
+  void *d_dldp = nullptr, *d_grad = nullptr, *d_stats = nullptr, *d_gs = nullptr, *d_apm = nullptr;
+  std::vector<float> apm(2 * (size_t)std::max(mats.n, 1));
+  for (int m = 0; m < mats.n; m++) {
+    apm[2 * m] = (float)(mat_param[4 * m + 3] + TPT_ADJ_ALPHA_H);
+    apm[2 * m + 1] = (float)(mat_param[4 * m + 3] - TPT_ADJ_ALPHA_H);
+  }
   auto cleanup = [&]() {
+    if (d_apm) (void)hipFree(d_apm);
     if (d_dldp) (void)hipFree(d_dldp);
     if (d_grad) (void)hipFree(d_grad);
     if (d_stats) (void)hipFree(d_stats);
@@ -359,6 +358,8 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   hipError_t e = hipMalloc(&d_dldp, npix * 12);
   if (e == hipSuccess) e = hipMalloc(&d_grad, (size_t)mats.n * 4 * 8 + 8);
   if (e == hipSuccess) e = hipMalloc(&d_stats, 64);
+  if (e == hipSuccess) e = hipMalloc(&d_apm, apm.size() * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_apm, apm.data(), apm.size() * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && gcap > 0) e = hipMalloc(&d_gs, (size_t)std::max(blocks, 1) * TPT_ADJ_BLOCK * gcap * 4);
   if (e == hipSuccess) e = hipMemcpyAsync(d_dldp, d_loss_d_pixel, npix * 12, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_grad, grad_params, (size_t)mats.n * 4 * 8, hipMemcpyHostToDevice, s);
@@ -387,6 +388,7 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   A.seed = cfg->seed;
   A.n_paths = n_paths;
   A.dldp = (const float*)d_dldp;
+  A.alpha_pm = (const float*)d_apm;
   A.grad = (double*)d_grad;
   A.stats = (uint64_t*)d_stats;
   hipEvent_t e0, e1;

@@ -258,6 +258,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate(WfParams W, AccumAr
 // Survivors are compacted into `active_out`.
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W, AdaptiveArgs A) {
   const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t kept = 0;  // samples this thread added to its pixels (the paths the render counts)
   for (uint32_t base = blockIdx.x * blockDim.x; base < A.n_active; base += stride) {
     uint32_t j = base + threadIdx.x;
     bool keep = false;
@@ -265,6 +266,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W
     if (j < A.n_active) {
       pix = A.active_in[j];
       uint32_t n = A.nsamp[pix];
+      const uint32_t n0 = n;
       double mean = A.welford[2 * pix], m2 = A.welford[2 * pix + 1];
       double r = 0, g = 0, b = 0, r2 = 0, g2 = 0, b2 = 0;
       bool stop = false;
@@ -290,12 +292,15 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W
       if (A.acc_sq) { A.acc_sq[3 * pix] += r2; A.acc_sq[3 * pix + 1] += g2; A.acc_sq[3 * pix + 2] += b2; }
       A.acc_w[pix] = (double)n;
       A.nsamp[pix] = n;
+      kept += n - n0;
       A.welford[2 * pix] = mean;
       A.welford[2 * pix + 1] = m2;
       keep = !stop;
     }
     wave_enqueue(keep, pix, A.active_out, A.n_active_out);
   }
+  for (int off = 32; off > 0; off >>= 1) kept += (uint32_t)__shfl_down((int)kept, off);
+  if (__lane_id() == 0 && kept) atomicAdd((unsigned long long*)&W.ps.stats[2], (unsigned long long)kept);
 }
 
 // ---------------------------------------------------------------- host-side launchers

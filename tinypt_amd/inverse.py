@@ -71,7 +71,7 @@ def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: in
     the mean wall time of a step (forward render + adjoint + update)."""
     x = torch.tensor(prob.init, dtype=torch.float64, requires_grad=True)
     opt = torch.optim.Adam([x], lr=lr)
-    losses, errs, times = [], [], []
+    losses, errs, f0_errs, alpha_errs, times = [], [], [], [], []
     truth = torch.from_numpy(prob.truth)
     for it in range(steps):
         t0 = time.perf_counter()
@@ -84,9 +84,15 @@ def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: in
             x.clamp_(0.01, 1.0)
         times.append(time.perf_counter() - t0)
         losses.append(float(loss))
-        errs.append(float((x.detach() - truth).abs().mean()))
+        err = (x.detach() - truth).abs()
+        errs.append(float(err.mean()))
+        f0_errs.append(float(err[:, :3].mean()))
+        alpha_errs.append(float(err[:, 3].mean()))
         if log and (it % 20 == 0 or it == steps - 1):
             log(f"step {it}: loss {losses[-1]:.6g} param error {errs[-1]:.4f} ({times[-1] * 1e3:.1f} ms)")
     prob.dscene.update_materials(prob.base)
-    return {"loss": losses, "param_error": errs, "s_per_iter": float(np.mean(times[1:] if steps > 1 else times)),
-            "params": x.detach().numpy().copy(), "init_error": float(np.abs(prob.init - prob.truth).mean())}
+    init_err = np.abs(prob.init - prob.truth)
+    return {"loss": losses, "param_error": errs, "f0_error": f0_errs, "alpha_error": alpha_errs,
+            "s_per_iter": float(np.mean(times[1:] if steps > 1 else times)), "params": x.detach().numpy().copy(),
+            "init_error": float(init_err.mean()), "init_f0_error": float(init_err[:, :3].mean()),
+            "init_alpha_error": float(init_err[:, 3].mean())}
