@@ -1,0 +1,47 @@
+// host_trace.hip -- test hook: the FP32 traversal (tpt_trace32.h) compiled for the host and run
+// over the product's host scene build, so CPU tests can compare it with the oracle's World::hit
+// without a GPU.  Not part of the product library; built next to it by tinypt_amd/build.py.
+#include <cmath>
+#include <vector>
+
+#include "scene_host.h"
+#include "tpt_trace32.h"
+
+using namespace tpt;
+
+extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const float* rays, float* t, int32_t* kind,
+                                   int32_t* inst, int32_t* prim, int32_t* flat) {
+  HostScene H;
+  if (!H.build(d)) return -1;
+  SceneF32 S{};
+  S.nodes = H.nodes2.data();
+  S.tris = H.tris32.data();
+  S.tri_mat = H.tri_mat32.data();
+  S.tri_orig = H.tri_orig.data();
+  S.tri_inst = H.flat ? H.tri_inst32.data() : nullptr;
+  S.inst = H.inst32.data();
+  S.tlas_items = H.tlas_items.data();
+  S.sph = H.sph.data();
+  S.sph_mat = H.sph_mat.data();
+  S.tlas_root = H.tlas_root;
+  S.n_spheres = (int32_t)H.sph_mat.size();
+  S.flat_inst = H.flat ? H.flat_inst : -1;
+  *flat = H.flat ? 1 : 0;
+  const int depth = H.max_tlas_depth + std::max(H.max_blas_depth, H.flat_depth) + 2;
+  std::vector<int32_t> stack((size_t)depth + 8);
+  LaneStack st{stack.data(), stack.data() + 4, 1, 4};  // 4 "LDS" entries, the rest overflow
+  for (int64_t i = 0; i < n; i++) {
+    const float* r = rays + 8 * i;
+    HitF32 h;
+    const bool f = trace_f32<false>(S, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), r[6], r[7], &h, st);
+    if (!f) {
+      t[i] = INFINITY; kind[i] = TPT_HIT_NONE; inst[i] = -1; prim[i] = -1;
+    } else if ((uint32_t)h.inst & TPT_ITEM_SPHERE) {
+      t[i] = h.t; kind[i] = TPT_HIT_SPHERE; inst[i] = (int32_t)((uint32_t)h.inst & ~TPT_ITEM_SPHERE); prim[i] = -1;
+    } else {
+      t[i] = h.t; kind[i] = TPT_HIT_TRI; inst[i] = (int32_t)((uint32_t)h.inst & ~TPT_HIT_FLAT);
+      prim[i] = H.tri_orig[(size_t)h.prim];
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,109 @@
+"""The FP32 traversal (tpt_trace32.h) built for the host (tests/native/libhost_trace.so) against
+the oracle's FP64 World::hit, on the CPU.
+
+Covers both top-level forms: small scenes whose instances are flattened into one world-space
+tree (Cornell box, material scene), and scenes traversed instance by instance (the displaced
+mesh is above the flattening limit when instanced 64 times).  The same (kind, instance,
+triangle) must come back on all rays but those whose answer FP32 cannot decide: equal-distance
+ties and rays within 1e-5 of an edge (the oracle itself flips on a nudged copy).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tinypt_amd import scenes
+
+from parity_scenes import PARITY_CASES
+
+LIB = os.path.join(os.path.dirname(__file__), "native", "libhost_trace.so")
+
+
+def _lib():
+    if not os.path.exists(LIB):
+        pytest.fail("tests/native/libhost_trace.so is not built (run __graft_entry__.build())")
+    return C.CDLL(LIB)
+
+
+def host_trace(sc, rays):
+    desc, keep = sc.to_desc()
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    n = len(r)
+    t = np.zeros(n, np.float32)
+    kind, inst, prim = (np.zeros(n, np.int32) for _ in range(3))
+    flat = C.c_int32()
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    rc = _lib().tpt_test_host_trace(C.byref(desc), C.c_int64(n), r.ctypes.data_as(C.POINTER(C.c_float)),
+                                    t.ctypes.data_as(C.POINTER(C.c_float)), ip(kind), ip(inst), ip(prim), C.byref(flat))
+    assert rc == 0
+    return dict(t=t, kind=kind, inst=inst, prim=prim), bool(flat.value)
+
+
+def _rays(rng, n, lo, hi, targets=None, tmax=1e30):
+    o = rng.uniform(lo, hi, (n, 3))
+    if targets is None:
+        d = rng.standard_normal((n, 3))
+    else:
+        d = targets[rng.integers(len(targets), size=n)] + rng.normal(0, 0.3, (n, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 8))
+    r[:, :3], r[:, 3:6], r[:, 6], r[:, 7] = o, d, 1e-4, tmax
+    return r.astype(np.float32).astype(np.float64)
+
+
+def _check(sc, rays, expect_flat):
+    g, flat = host_trace(sc, rays)
+    assert flat == expect_flat
+    orc = oracle.OracleScene(sc)
+    r = orc.trace(rays, threads=8)
+    same = (g["kind"] == r["kind"]) & (g["inst"] == r["inst"]) & (g["prim"] == r["prim"])
+    unexplained = 0
+    for i in np.nonzero(~same)[0]:
+        if g["kind"][i] and r["kind"][i] and abs(float(g["t"][i]) - r["t"][i]) <= 1e-4 * max(1.0, r["t"][i]):
+            continue
+        ok = False
+        for ax in range(3):
+            for s in (-1e-5, 1e-5):
+                q = rays[i:i + 1].copy()
+                q[0, ax] += s * max(1.0, abs(q[0, ax]))
+                rq = orc.trace(q, threads=1)
+                ok |= rq["kind"][0] == g["kind"][i] and rq["inst"][0] == g["inst"][i] and rq["prim"][0] == g["prim"][i]
+        unexplained += not ok
+    hit = same & (r["kind"] != 0)
+    rel = np.abs(g["t"][hit] - r["t"][hit]) / np.maximum(r["t"][hit], 1.0)
+    print(f"agree {same.mean():.6f}, unexplained {unexplained}, t rel err p99.9 {np.quantile(rel, 0.999):.2e}")
+    assert unexplained <= 1e-4 * len(rays)
+    assert np.quantile(rel, 0.999) < 1e-4
+
+
+def test_flattened_cornell():
+    rng = np.random.default_rng(1)
+    _check(scenes.cornell_box(16, 16), _rays(rng, 40_000, (-0.95, 0.05, -0.95), (0.95, 1.95, 0.95)), True)
+
+
+def test_flattened_material_scene_with_spheres_and_tmax():
+    rng = np.random.default_rng(2)
+    sc = PARITY_CASES["material_zoo"][0]()
+    targets = np.array([[-1.2, 0.4, 0.0], [-0.4, 0.4, 0.3], [0.4, 0.4, 0.0], [1.2, 0.4, 0.3], [0.0, 0.25, -1.0]])
+    _check(sc, _rays(rng, 40_000, (-3, 0.05, -3), (3, 3, 3), targets, tmax=3.0), True)
+
+
+def test_two_level_instanced():
+    rng = np.random.default_rng(3)
+    sc = scenes.instanced_scene(16, 16, n_variants=2, n_instances=64, subdivisions=6, box=3.0, env_size=(8, 4))
+    targets = np.array([inst.xform.t for inst in sc.world.instances])
+    _check(sc, _rays(rng, 20_000, (-4.5, -4.5, -4.5), (4.5, 4.5, 4.5), targets), False)
+
+
+def test_flattening_keeps_mirrored_instance_normals():
+    """An instance with det(A) < 0 flips e1 x e2 in world space; the hit must still report the
+    reference's instance and triangle (normal orientation is checked on the GPU renders)."""
+    rng = np.random.default_rng(4)
+    b = scenes.SceneBuilder()
+    from tinypt_amd.scene import Camera, EnvMap, Lambert, Transform
+    b.mesh_instance(scenes.unit_cube_tris(), scenes.compose_ops([Transform.scale((-1.0, 0.5, 1.0))]), Lambert((0.5,) * 3))
+    b.mesh_instance(scenes.unit_cube_tris(), scenes.compose_ops([Transform.translate((2.0, 0, 0))]), Lambert((0.5,) * 3))
+    sc = b.build(Camera.look_at((0, 0, 5), (0, 0, 0), (0, 1, 0), 40, 1), EnvMap.constant((1.0, 1.0, 1.0)))
+    _check(sc, _rays(rng, 20_000, (-3, -3, -3), (3, 3, 3), np.array([[0.0, 0, 0], [2.0, 0, 0]])), True)

@@ -66,7 +66,7 @@ struct tpt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   // wavefront scratch (grow-only)
-  DevBuf ps_o, ps_d, ps_beta, ps_L, ps_rng, ps_pix, ps_hit, sh_o, sh_d, sh_c, q0, q1, sq, counters, stats, gstack;
+  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o, sh_d, sh_c, sh_l, sh_lv, counters, stats, gstack;
   // adaptive / plain per-pixel state
   DevBuf act0, act1, nsamp, welford, rngpix, done;
   size_t ps_cap = 0;
@@ -75,7 +75,7 @@ struct tpt_ctx {
 struct tpt_scene {
   tpt_ctx* ctx = nullptr;  // null for a host-only scene (tpt_scene_build_host)
   HostScene host;
-  DevBuf nodes2, tris32, tri_mat32, tri_orig, inst32, tlas_items, sph64, sph_mat;
+  DevBuf nodes2, tris32, tri_mat32, tri_orig, tri_inst, inst32, tlas_items, sph64, sph_mat;
   DevBuf ref_nodes, ref_indices, tri_v, tri_mat, meshes, inst64;
   DevBuf mat_kind, mat_p32, mat_p64;
   DevBuf l32, cdf32, l64, cdf64;
@@ -170,6 +170,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   HIPC(upload(sc->tris32, H.tris32, s));
   HIPC(upload(sc->tri_mat32, H.tri_mat32, s));
   HIPC(upload(sc->tri_orig, H.tri_orig, s));
+  if (H.flat) HIPC(upload(sc->tri_inst, H.tri_inst32, s));
   HIPC(upload(sc->inst32, H.inst32, s));
   HIPC(upload(sc->tlas_items, H.tlas_items, s));
   HIPC(upload(sc->sph64, sph64, s));
@@ -214,6 +215,8 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   S.tris = sc->tris32.as<TriF32>();
   S.tri_mat = sc->tri_mat32.as<int32_t>();
   S.tri_orig = sc->tri_orig.as<int32_t>();
+  S.tri_inst = H.flat ? sc->tri_inst.as<int32_t>() : nullptr;
+  S.flat_inst = H.flat ? H.flat_inst : -1;
   S.inst = sc->inst32.as<InstF32>();
   S.tlas_items = sc->tlas_items.as<int32_t>();
   S.sph = sc->sph64.as<double>();
@@ -246,7 +249,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   C.lens_radius = (float)H.cam.lens_radius;
   C.focus_dist = (float)H.cam.focus_dist;
   HIPC(hipStreamSynchronize(s));
-  for (DevBuf* b : {&sc->nodes2, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->inst32, &sc->tlas_items, &sc->sph64,
+  for (DevBuf* b : {&sc->nodes2, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->tri_inst, &sc->inst32, &sc->tlas_items, &sc->sph64,
                     &sc->sph_mat, &sc->ref_nodes, &sc->ref_indices, &sc->tri_v, &sc->tri_mat, &sc->meshes, &sc->inst64,
                     &sc->mat_kind, &sc->mat_p32, &sc->mat_p64, &sc->l32, &sc->cdf32, &sc->l64, &sc->cdf64, &sc->env_rgb32,
                     &sc->alias_prob, &sc->alias_idx, &sc->cell_pdf, &sc->env_rgb64, &sc->row_cdf, &sc->col_cdf})
@@ -377,7 +380,7 @@ int check_cfg(const tpt_render_config* c) {
 
 // Deepest traversal stack any ray can need: one entry per TLAS level, the instance sentinel,
 // one entry per BLAS level, plus one of slack.
-int stack_need(const HostScene& H) { return H.max_tlas_depth + H.max_blas_depth + 2; }
+int stack_need(const HostScene& H) { return H.max_tlas_depth + std::max(H.max_blas_depth, H.flat_depth) + 2; }
 
 // Global overflow entries per lane beyond the TPT_WF_STACK entries kept in LDS.
 int needed_gstack(const HostScene& H) {
@@ -387,19 +390,19 @@ int needed_gstack(const HostScene& H) {
 
 int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
   if (P > c->ps_cap) {
-    HIPC(c->ps_o.ensure(P * 16));
-    HIPC(c->ps_d.ensure(P * 16));
-    HIPC(c->ps_beta.ensure(P * 16));
-    HIPC(c->ps_L.ensure(P * 16));
-    HIPC(c->ps_rng.ensure(P * 8));
-    HIPC(c->ps_pix.ensure(P * 4));
+    for (int b = 0; b < 2; b++) {
+      HIPC(c->ps_o[b].ensure(P * 16));
+      HIPC(c->ps_d[b].ensure(P * 16));
+      HIPC(c->ps_beta[b].ensure(P * 16));
+      HIPC(c->ps_rng[b].ensure(P * 8));
+    }
     HIPC(c->ps_hit.ensure(P * 16));
+    HIPC(c->ps_L.ensure(P * 16));
     HIPC(c->sh_o.ensure(P * 32));
     HIPC(c->sh_d.ensure(P * 32));
     HIPC(c->sh_c.ensure(P * 32));
-    HIPC(c->q0.ensure(P * 4));
-    HIPC(c->q1.ensure(P * 4));
-    HIPC(c->sq.ensure(P * 4));
+    HIPC(c->sh_l.ensure(P * 16));
+    HIPC(c->sh_lv.ensure(P * 8));
     c->ps_cap = P;
   }
   HIPC(c->counters.ensure(TPT_NUM_CNT * 4));
@@ -417,19 +420,19 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   W.mats = sc->mats;
   W.cam = sc->cam32;
   PathState& ps = W.ps;
-  ps.o = c->ps_o.as<float4>();
-  ps.d = c->ps_d.as<float4>();
-  ps.beta = c->ps_beta.as<float4>();
-  ps.L = c->ps_L.as<float4>();
-  ps.rng = c->ps_rng.as<uint64_t>();
-  ps.pix = c->ps_pix.as<uint32_t>();
+  for (int b = 0; b < 2; b++) {
+    ps.o[b] = c->ps_o[b].as<float4>();
+    ps.d[b] = c->ps_d[b].as<float4>();
+    ps.beta[b] = c->ps_beta[b].as<float4>();
+    ps.rng[b] = c->ps_rng[b].as<uint64_t>();
+  }
   ps.hit = c->ps_hit.as<int4>();
+  ps.L = c->ps_L.as<float4>();
   ps.sh_o = c->sh_o.as<float4>();
   ps.sh_d = c->sh_d.as<float4>();
   ps.sh_c = c->sh_c.as<float4>();
-  ps.q[0] = c->q0.as<int32_t>();
-  ps.q[1] = c->q1.as<int32_t>();
-  ps.sq = c->sq.as<int32_t>();
+  ps.sh_l = c->sh_l.as<float4>();
+  ps.sh_lv = c->sh_lv.as<float2>();
   ps.counters = c->counters.as<uint32_t>();
   ps.stats = c->stats.as<uint64_t>();
   W.gstack = gcap > 0 ? c->gstack.as<int32_t>() : nullptr;
@@ -480,6 +483,7 @@ int wf_bounces(tpt_ctx* c, const WfParams& W, uint32_t P, EventTimer& tt, uint64
     HIPC(tt.mark());
     (*launches)++;
     HIPC(wf_launch_shade(W, cur, P, s));
+    if (W.lights.n > 0) HIPC(wf_launch_light_selfocc(W, P, s));
     HIPC(wf_launch_shadow(W, P, s));
     cur ^= 1;
   }
@@ -491,7 +495,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   hipStream_t s = c->stream;
   const uint32_t npix = (uint32_t)cfg->width * (uint32_t)cfg->height;
   const uint32_t spp = (uint32_t)cfg->spp;
-  const uint32_t target = 1u << 22;  // paths in flight
+  const uint32_t target = 1u << 24;  // paths in flight (about 3.8 GB of path state)
   uint32_t k = cfg->samples_per_pass > 0 ? (uint32_t)cfg->samples_per_pass : std::max(1u, target / std::max(npix, 1u));
   k = std::max(1u, std::min(k, std::max(spp, 1u)));
   uint32_t pix_chunk = npix;

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(TPT_TR_BLOCK) k_trace_closest(SceneF32 S, int6
     } else {
       t_out[i] = h.t;
       kind[i] = TPT_HIT_TRI;
-      inst[i] = h.inst;
+      inst[i] = (int32_t)((uint32_t)h.inst & ~TPT_HIT_FLAT);
       prim[i] = S.tri_orig[h.prim];
     }
   }

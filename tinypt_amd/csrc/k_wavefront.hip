@@ -19,15 +19,25 @@
 namespace tpt {
 
 // ---------------------------------------------------------------- queue compaction
-__device__ __forceinline__ void wave_enqueue(bool pred, int32_t val, int32_t* __restrict__ q, uint32_t* __restrict__ count) {
-  uint64_t mask = __ballot(pred);
-  if (mask == 0) return;
-  uint32_t lane = __lane_id();
-  uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-  base = (uint32_t)__shfl((int)base, (int)leader);
-  if (pred) q[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = val;
+// Position of this lane's entry in a queue when `pred` holds: a ballot per wave, the waves'
+// counts combined in LDS, and one atomic per block, lanes ordered by block thread id.  Every
+// thread of the block must call it (it synchronises the block); `sh` is 8 words of LDS.
+__device__ __forceinline__ uint32_t block_slot(bool pred, uint32_t* __restrict__ count, uint32_t* sh) {
+  const uint64_t mask = __ballot(pred);
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < TPT_WF_BLOCK / 64; w++) tot += sh[w];
+    sh[7] = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = sh[7];
+  for (uint32_t w = 0; w < wave; w++) base += sh[w];
+  const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  __syncthreads();  // sh is reused by the next call
+  return pos;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -49,13 +59,11 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
     RngF32 rng(seed_for(x, y, s) ^ W.seed);
     f3 origin, dir;
     camera_ray32(W.cam, x, y, W.inv_w, W.inv_h, rng, &origin, &dir);
-    W.ps.o[i] = make_float4(origin.x, origin.y, origin.z, 0.f);
-    W.ps.d[i] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
-    W.ps.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+    W.ps.o[0][i] = make_float4(origin.x, origin.y, origin.z, 0.f);
+    W.ps.d[0][i] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+    W.ps.beta[0][i] = make_float4(1.f, 1.f, 1.f, __int_as_float((int32_t)i));
+    W.ps.rng[0][i] = rng.p.state;
     W.ps.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    W.ps.rng[i] = rng.p.state;
-    W.ps.pix[i] = pix;
-    W.ps.q[0][i] = (int32_t)i;
   }
   if (gtid() == 0) {
     W.ps.counters[TPT_CNT_Q0] = P;
@@ -67,7 +75,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_extend(WfParams W, int cur) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   const uint32_t n = W.ps.counters[cur ? TPT_CNT_Q1 : TPT_CNT_Q0];
-  const int32_t* __restrict__ q = W.ps.q[cur];
+  const float4* __restrict__ po = W.ps.o[cur];
+  const float4* __restrict__ pd = W.ps.d[cur];
   LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
                TPT_WF_STACK};
   if (gtid() == 0) {
@@ -76,136 +85,177 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_extend(WfParams W, int cur) {
     atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
   }
   for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
-    int32_t slot = q[i];
-    float4 o4 = W.ps.o[slot], d4 = W.ps.d[slot];
+    float4 o4 = po[i], d4 = pd[i];
     HitF32 h;
     trace_f32<false>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, 1e30f, &h, st);
-    W.ps.hit[slot] = make_int4(__float_as_int(h.t), h.inst, h.prim, 0);
+    W.ps.hit[i] = make_int4(__float_as_int(h.t), h.inst, h.prim, 0);
   }
 }
 
-__global__ void __launch_bounds__(TPT_WF_BLOCK) k_shade(WfParams W, int cur) {
+__device__ __forceinline__ void add_radiance(float4* L, int32_t id, f3 c) {
+  float4 l = L[id];
+  L[id] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+}
+
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shade(WfParams W, int cur) {
+  __shared__ uint32_t s_slot[8];
   const uint32_t n = W.ps.counters[cur ? TPT_CNT_Q1 : TPT_CNT_Q0];
-  const int32_t* __restrict__ qin = W.ps.q[cur];
-  int32_t* __restrict__ qout = W.ps.q[cur ^ 1];
+  const int nxt = cur ^ 1;
   uint32_t* cout = &W.ps.counters[cur ? TPT_CNT_Q0 : TPT_CNT_Q1];
+  uint32_t* csh = &W.ps.counters[TPT_CNT_SH];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     bool alive = false, has_shadow = false;
-    int32_t slot = 0;
+    // path state (valid when i < n)
+    f3 o = mk(0.f, 0.f, 0.f), d = o, beta = o;
+    float last_pdf = 0.f;
+    uint32_t flags = 0;
+    int32_t id = 0;
+    RngF32 rng(1);
+    int4 h4 = make_int4(0, -1, 0, 0);
     if (i < n) {
-      slot = qin[i];
-      float4 o4 = W.ps.o[slot], d4 = W.ps.d[slot], b4 = W.ps.beta[slot], l4 = W.ps.L[slot];
-      int4 h4 = W.ps.hit[slot];
-      f3 o = f3of(o4), d = f3of(d4), beta = f3of(b4), L = f3of(l4);
-      float last_pdf = o4.w;
-      uint32_t flags = __float_as_uint(d4.w);
-      int bounce = (int)(flags & 0xffu);
-      bool last_nd = (flags & 0x100u) != 0;
-      RngF32 rng(1);
-      rng.p.state = W.ps.rng[slot];
-      float4 sh_c0 = make_float4(0.f, 0.f, 0.f, 0.f), sh_c1 = sh_c0;
+      float4 o4 = W.ps.o[cur][i], d4 = W.ps.d[cur][i], b4 = W.ps.beta[cur][i];
+      h4 = W.ps.hit[i];
+      o = f3of(o4);
+      d = f3of(d4);
+      beta = f3of(b4);
+      id = __float_as_int(b4.w);
+      last_pdf = o4.w;
+      flags = __float_as_uint(d4.w);
+      rng.p.state = W.ps.rng[cur][i];
+    }
+    const int bounce = (int)(flags & 0xffu);
+    const bool last_nd = (flags & 0x100u) != 0;
+    // ---- phase 1: miss, Russian roulette, emission, next-event estimation
+    float4 sh_c0 = make_float4(0.f, 0.f, 0.f, 0.f), sh_c1 = sh_c0;
+    float4 sh_o0 = sh_c0, sh_d0 = sh_c0, sh_o1 = sh_c0, sh_d1 = sh_c0, sh_l = sh_c0;
+    float2 sh_lv = make_float2(0.f, 0.f);
+    bool go = false;  // hit a scattering surface: sample the BSDF in phase 2
+    f3 p = mk(0.f, 0.f, 0.f), ng = p;
+    int mat = 0, kind = 0;
+    float4 mp = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n) {
       if (h4.y == -1) {
-        // ---- miss (integrator.rs:71-83)
+        // miss (integrator.rs:71-83)
         f3 bg = W.env.kind == TPT_ENV_IMAGE ? env_lookup32(W.env, d) : sky32(d);
         f3 c = had(beta, bg);
         if (W.env.kind == TPT_ENV_IMAGE && last_nd) c = c * mis_weight(last_pdf, env_pdf32(W.env, d));
-        L = L + clamp_luminance(c, (float)TPT_FIREFLY_CLAMP);
+        add_radiance(W.ps.L, id, clamp_luminance(c, (float)TPT_FIREFLY_CLAMP));
       } else {
-        bool go = true;
+        go = true;
         if (bounce >= W.rr_start) {  // integrator.rs:89-95
-          float p = fmaxf(fminf(fmaxf(fmaxf(beta.x, beta.y), beta.z), 0.95f), 0.05f);
-          if (rng.uni() > p) go = false;
-          else beta = beta / p;
+          float q = fmaxf(fminf(fmaxf(fmaxf(beta.x, beta.y), beta.z), 0.95f), 0.05f);
+          if (rng.uni() > q) go = false;
+          else beta = beta / q;
         }
         if (go) {
           const float t = __int_as_float(h4.x);
-          f3 p, ng;
-          int mat;
           hit_geometry32(W.S, h4.y, h4.z, t, o, d, &p, &ng, &mat);
-          const int kind = W.mats.kind[mat];
-          const float4 mp = W.mats.p32[mat];
+          kind = W.mats.kind[mat];
+          mp = W.mats.p32[mat];
           const f3 mc = f3of(mp);
           if (kind == TPT_MAT_DIFFUSE_LIGHT) {  // integrator.rs:100-104
-            L = L + clamp_luminance(had(beta, mc), (float)TPT_FIREFLY_CLAMP);
-          } else {
-            f3 n = dot(ng, d) < 0.f ? ng : -ng;  // oriented_normal (integrator.rs:138-140)
-            if (!mat_is_delta(kind)) {
-              f3 wo = vnorm(-d);
-              if (W.env.kind == TPT_ENV_IMAGE) {  // nee_environment (integrator.rs:148-172)
-                f3 wi, li;
-                float pdf_env;
-                env_sample32(W.env, rng, &wi, &li, &pdf_env);
-                float cs = fmaxf(dot(n, wi), 0.f);
-                if (cs > 0.f && pdf_env > 0.f && (li.x != 0.f || li.y != 0.f || li.z != 0.f)) {
-                  f3 f;
-                  float pdf_b;
-                  bsdf_eval(kind, mc, mp.w, wo, wi, n, &f, &pdf_b);
-                  f3 c = had(had(beta, f), li) * (cs * mis_weight(pdf_env, pdf_b) / pdf_env);
-                  if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
-                    f3 so = p + (float)TPT_RAY_EPSILON * wi;
-                    W.ps.sh_o[2 * slot] = make_float4(so.x, so.y, so.z, 1e30f);
-                    W.ps.sh_d[2 * slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
-                    sh_c0 = make_float4(c.x, c.y, c.z, 1.f);
-                    has_shadow = true;
-                  }
-                }
-              }
-              LightSample32 ls;  // nee_area_light (integrator.rs:178-213)
-              if (sample_light32(W.lights, rng, p, &ls)) {
-                f3 to_light = ls.pos - p;
-                float dist = sqrtf(dot(to_light, to_light));
-                if (dist > 1e-6f) {
-                  f3 wi = to_light / dist;
-                  float cs = fmaxf(dot(n, wi), 0.f);
-                  if (cs > 0.f) {
-                    f3 f;
-                    float pdf_b;
-                    bsdf_eval(kind, mc, mp.w, wo, wi, n, &f, &pdf_b);
-                    f3 c = had(had(beta, f), ls.emit) * (cs * mis_weight(ls.pdf, pdf_b) / ls.pdf);
-                    if ((c.x != 0.f || c.y != 0.f || c.z != 0.f) &&
-                        !light_self_occluded64(W.S64, W.lights, ls.idx, p.x, p.y, p.z, ls.u, ls.v)) {
-                      f3 so = p + (float)TPT_RAY_EPSILON * wi;
-                      float tmax = light_shadow_tmax32(dist);
-                      W.ps.sh_o[2 * slot + 1] = make_float4(so.x, so.y, so.z, tmax);
-                      W.ps.sh_d[2 * slot + 1] = make_float4(wi.x, wi.y, wi.z, 0.f);
-                      sh_c1 = make_float4(c.x, c.y, c.z, 1.f);
-                      has_shadow = true;
-                    }
-                  }
+            add_radiance(W.ps.L, id, clamp_luminance(had(beta, mc), (float)TPT_FIREFLY_CLAMP));
+            go = false;
+          } else if (!mat_is_delta(kind)) {
+            f3 n_o = dot(ng, d) < 0.f ? ng : -ng;  // oriented_normal (integrator.rs:138-140)
+            f3 wo = vnorm(-d);
+            if (W.env.kind == TPT_ENV_IMAGE) {  // nee_environment (integrator.rs:148-172)
+              f3 wi, li;
+              float pdf_env;
+              env_sample32(W.env, rng, &wi, &li, &pdf_env);
+              float cs = fmaxf(dot(n_o, wi), 0.f);
+              if (cs > 0.f && pdf_env > 0.f && (li.x != 0.f || li.y != 0.f || li.z != 0.f)) {
+                f3 f;
+                float pdf_b;
+                bsdf_eval(kind, mc, mp.w, wo, wi, n_o, &f, &pdf_b);
+                f3 c = had(had(beta, f), li) * (cs * mis_weight(pdf_env, pdf_b) / pdf_env);
+                if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
+                  f3 so = p + (float)TPT_RAY_EPSILON * wi;
+                  sh_o0 = make_float4(so.x, so.y, so.z, 1e30f);
+                  sh_d0 = make_float4(wi.x, wi.y, wi.z, 0.f);
+                  sh_c0 = make_float4(c.x, c.y, c.z, 1.f);
+                  has_shadow = true;
                 }
               }
             }
-            if (bounce + 1 < W.max_bounces) {  // the scattered ray of the last bounce is never traced
-              BsdfSample<float> bs;
-              if (bsdf_sample(kind, mc, mp.w, d, p, ng, t, rng, &bs)) {
-                beta = had(beta, bs.weight);
-                o = bs.o;
-                d = bs.d;
-                last_pdf = bs.pdf;
-                last_nd = !bs.is_delta;
-                alive = true;
+            LightSample32 ls;  // nee_area_light (integrator.rs:178-213)
+            if (sample_light32(W.lights, rng, p, &ls)) {
+              f3 to_light = ls.pos - p;
+              float dist = sqrtf(dot(to_light, to_light));
+              if (dist > 1e-6f) {
+                f3 wi = to_light / dist;
+                float cs = fmaxf(dot(n_o, wi), 0.f);
+                if (cs > 0.f) {
+                  f3 f;
+                  float pdf_b;
+                  bsdf_eval(kind, mc, mp.w, wo, wi, n_o, &f, &pdf_b);
+                  f3 c = had(had(beta, f), ls.emit) * (cs * mis_weight(ls.pdf, pdf_b) / ls.pdf);
+                  if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
+                    f3 so = p + (float)TPT_RAY_EPSILON * wi;
+                    sh_o1 = make_float4(so.x, so.y, so.z, light_shadow_tmax32(dist));
+                    sh_d1 = make_float4(wi.x, wi.y, wi.z, 0.f);
+                    sh_c1 = make_float4(c.x, c.y, c.z, 1.f);
+                    sh_l = make_float4(p.x, p.y, p.z, ls.u);  // for k_light_selfocc
+                    sh_lv = make_float2(ls.v, __int_as_float(ls.idx));
+                    has_shadow = true;
+                  }
+                }
               }
             }
           }
         }
       }
-      if (has_shadow) {
-        W.ps.sh_c[2 * slot] = sh_c0;
-        W.ps.sh_c[2 * slot + 1] = sh_c1;
-      }
-      W.ps.L[slot] = make_float4(L.x, L.y, L.z, 0.f);
-      if (alive) {
-        flags = (uint32_t)(bounce + 1) | (last_nd ? 0x100u : 0u);
-        W.ps.o[slot] = make_float4(o.x, o.y, o.z, last_pdf);
-        W.ps.d[slot] = make_float4(d.x, d.y, d.z, __uint_as_float(flags));
-        W.ps.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
-        W.ps.rng[slot] = rng.p.state;
+    }
+    const uint32_t j = block_slot(has_shadow, csh, s_slot);
+    if (has_shadow) {
+      sh_d0.w = __int_as_float(id);
+      W.ps.sh_o[2 * j] = sh_o0;
+      W.ps.sh_d[2 * j] = sh_d0;
+      W.ps.sh_c[2 * j] = sh_c0;
+      W.ps.sh_o[2 * j + 1] = sh_o1;
+      W.ps.sh_d[2 * j + 1] = sh_d1;
+      W.ps.sh_c[2 * j + 1] = sh_c1;
+      W.ps.sh_l[j] = sh_l;
+      W.ps.sh_lv[j] = sh_lv;
+    }
+    // ---- phase 2: BSDF sampling; survivors go densely into the next buffer set
+    bool bsdf_nd = false;
+    if (go && bounce + 1 < W.max_bounces) {  // the scattered ray of the last bounce is never traced
+      BsdfSample<float> bs;
+      if (bsdf_sample(kind, f3of(mp), mp.w, d, p, ng, __int_as_float(h4.x), rng, &bs)) {
+        beta = had(beta, bs.weight);
+        o = bs.o;
+        d = bs.d;
+        last_pdf = bs.pdf;
+        bsdf_nd = !bs.is_delta;
+        alive = true;
       }
     }
-    wave_enqueue(alive, slot, qout, cout);
-    wave_enqueue(has_shadow, slot, W.ps.sq, &W.ps.counters[TPT_CNT_SH]);
+    const uint32_t k = block_slot(alive, cout, s_slot);
+    if (alive) {
+      uint32_t nflags = (uint32_t)(bounce + 1) | (bsdf_nd ? 0x100u : 0u);
+      W.ps.o[nxt][k] = make_float4(o.x, o.y, o.z, last_pdf);
+      W.ps.d[nxt][k] = make_float4(d.x, d.y, d.z, __uint_as_float(nflags));
+      W.ps.beta[nxt][k] = make_float4(beta.x, beta.y, beta.z, __int_as_float(id));
+      W.ps.rng[nxt][k] = rng.p.state;
+    }
+  }
+}
+
+// The reference's light shadow ray can be blocked by the light itself; that decision is taken
+// in FP64 (tpt_selfocc64.h) here, in a kernel of its own so the FP64 registers do not weigh on
+// k_shade.  A self-occluded ray is dropped before k_shadow traces it.
+__global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
+  const uint32_t n = W.ps.counters[TPT_CNT_SH];
+  for (uint32_t j = gtid(); j < n; j += gridDim.x * blockDim.x) {
+    float4 c = W.ps.sh_c[2 * j + 1];
+    if (c.w == 0.f) continue;
+    float4 l = W.ps.sh_l[j];
+    float2 lv = W.ps.sh_lv[j];
+    if (light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x))
+      W.ps.sh_c[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -215,19 +265,20 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W) {
   LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
                TPT_WF_STACK};
   uint32_t traced = 0;
-  for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
-    int32_t slot = W.ps.sq[i];
+  for (uint32_t j = gtid(); j < n; j += gridDim.x * blockDim.x) {
     f3 add = mk(0.f, 0.f, 0.f);
+    int32_t id = 0;
     for (int k = 0; k < 2; k++) {
-      float4 c = W.ps.sh_c[2 * slot + k];
+      float4 d4 = W.ps.sh_d[2 * j + k];
+      if (k == 0) id = __float_as_int(d4.w);
+      float4 c = W.ps.sh_c[2 * j + k];
       if (c.w == 0.f) continue;
-      float4 o4 = W.ps.sh_o[2 * slot + k], d4 = W.ps.sh_d[2 * slot + k];
+      float4 o4 = W.ps.sh_o[2 * j + k];
       HitF32 h;
       traced++;
       if (!trace_f32<true>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, o4.w, &h, st)) add = add + f3of(c);
     }
-    float4 l4 = W.ps.L[slot];
-    W.ps.L[slot] = make_float4(l4.x + add.x, l4.y + add.y, l4.z + add.z, 0.f);
+    if (add.x != 0.f || add.y != 0.f || add.z != 0.f) add_radiance(W.ps.L, id, add);
   }
   // one atomic per wave for the shadow-ray count
   uint32_t tot = traced;
@@ -257,6 +308,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate(WfParams W, AccumAr
 // Welford on luminance, stop at the reference's criterion; samples past the stop are dropped.
 // Survivors are compacted into `active_out`.
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W, AdaptiveArgs A) {
+  __shared__ uint32_t s_slot[8];
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t kept = 0;  // samples this thread added to its pixels (the paths the render counts)
   for (uint32_t base = blockIdx.x * blockDim.x; base < A.n_active; base += stride) {
@@ -297,16 +349,20 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W
       A.welford[2 * pix + 1] = m2;
       keep = !stop;
     }
-    wave_enqueue(keep, pix, A.active_out, A.n_active_out);
+    const uint32_t q = block_slot(keep, A.n_active_out, s_slot);
+    if (keep) A.active_out[q] = pix;
   }
   for (int off = 32; off > 0; off >>= 1) kept += (uint32_t)__shfl_down((int)kept, off);
   if (__lane_id() == 0 && kept) atomicAdd((unsigned long long*)&W.ps.stats[2], (unsigned long long)kept);
 }
 
 // ---------------------------------------------------------------- host-side launchers
+// Grid of at most TPT_WF_MAX_BLOCKS blocks: the kernels loop over their queues, so the grid only
+// needs to fill the chip (256 CUs x 8 blocks); larger grids cost dispatch time per bounce,
+// most of all for launches whose queues have shrunk to a few paths.
 static inline uint32_t grid_for(uint32_t n) {
   uint32_t b = (n + TPT_WF_BLOCK - 1) / TPT_WF_BLOCK;
-  return b < 1 ? 1 : (b > 16384 ? 16384 : b);
+  return b < 1 ? 1 : (b > TPT_WF_MAX_BLOCKS ? TPT_WF_MAX_BLOCKS : b);
 }
 
 hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s) {
@@ -319,6 +375,10 @@ hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStrea
 }
 hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
   hipLaunchKernelGGL(k_shade, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  return hipGetLastError();
+}
+hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL(k_light_selfocc, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W);
   return hipGetLastError();
 }
 hipError_t wf_launch_shadow(const WfParams& W, uint32_t max_n, hipStream_t s) {

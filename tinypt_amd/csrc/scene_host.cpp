@@ -274,6 +274,10 @@ static void mat_vec(const double* m, const double* v, double* out) {
   out[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
   out[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
 }
+
+static double mat_det(const double* m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
 static double lum(const double* c) { return 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]; }
 
 // ------------------------------------------------------------------ build
@@ -449,6 +453,77 @@ bool HostScene::build(const tpt_scene_desc* d) {
     item_box.push_back(wb);
     item_code.push_back(i);
   }
+  // ---- small scenes: every instance's triangles in world space under one tree (own design).
+  // The top level then holds this tree and the spheres, and a ray enters no instance: no
+  // per-instance transform, no per-instance BLAS roots.  Hits still report the original
+  // instance and triangle, and ties between instances keep the reference's order (the
+  // triangle's instance gives its tie key).
+  {
+    int64_t total = 0;
+    for (int i = 0; i < ni; i++) {
+      const InstF64& x = inst64[(size_t)i];
+      if (x.mesh >= 0 && x.mesh < nmesh) total += mesh_tri_begin[(size_t)x.mesh + 1] - mesh_tri_begin[(size_t)x.mesh];
+    }
+    flat = total > 0 && total <= kFlatMaxTris;
+    if (flat) {
+      std::vector<double> wv;  // 9 per flattened triangle, world space
+      std::vector<int32_t> f_inst, f_orig, f_mat;
+      wv.reserve(9 * (size_t)total);
+      for (int i = 0; i < ni; i++) {
+        const InstF64& x = inst64[(size_t)i];
+        if (x.mesh < 0 || x.mesh >= nmesh) continue;
+        const bool flip = mat_det(x.a) < 0.0;  // (A e1) x (A e2) = det(A) A^-T (e1 x e2)
+        const int64_t b0 = mesh_tri_begin[(size_t)x.mesh], b1 = mesh_tri_begin[(size_t)x.mesh + 1];
+        for (int64_t t = b0; t < b1; t++) {
+          for (int v = 0; v < 3; v++) {
+            double q[3];
+            mat_vec(x.a, &tri_v[9 * (size_t)t + 3 * v], q);  // Transform::apply_point (transform.rs:75-78)
+            for (int k = 0; k < 3; k++) wv.push_back(q[k] + x.t[k]);
+          }
+          f_inst.push_back(i | (flip ? (int32_t)TPT_TRI_FLIP : 0));
+          f_orig.push_back((int32_t)(t - b0));
+          f_mat.push_back(x.mat_override >= 0 ? x.mat_override : tri_mat[(size_t)t]);
+        }
+      }
+      std::vector<DAabb> fb((size_t)total);
+      for (int64_t t = 0; t < total; t++) fb[(size_t)t] = triangle_bounds(&wv[9 * (size_t)t]);
+      RefBvh fr = build_ref_bvh(fb, 4);
+      flat_depth = fr.depth;
+      const int64_t base = (int64_t)tris32.size();
+      tris32.resize((size_t)(base + total));
+      tri_orig.resize((size_t)(base + total));
+      tri_mat32.resize((size_t)(base + total));
+      tri_inst32.assign((size_t)(base + total), 0);
+      for (size_t k = 0; k < fr.indices.size(); k++) {
+        const size_t t = (size_t)fr.indices[k];
+        const double* v = &wv[9 * t];
+        TriF32& o = tris32[(size_t)base + k];
+        for (int c = 0; c < 3; c++) {
+          o.v0[c] = (float)v[c];
+          o.e1[c] = (float)(v[3 + c] - v[c]);
+          o.e2[c] = (float)(v[6 + c] - v[c]);
+        }
+        o.v0[3] = o.e1[3] = o.e2[3] = 0.f;
+        tri_orig[(size_t)base + k] = f_orig[t];
+        tri_mat32[(size_t)base + k] = f_mat[t];
+        tri_inst32[(size_t)base + k] = f_inst[t];
+      }
+      InstF32 y{};
+      y.w2o[0] = y.w2o[5] = y.w2o[10] = 1.f;  // identity world->object rows
+      y.nrm[0] = y.nrm[5] = y.nrm[10] = 1.f;
+      y.root = to_node2(fr, 0, base, nodes2, error);
+      if (!error.empty()) return false;
+      y.mat_override = -1;
+      y.mesh = -1;
+      flat_inst = (int32_t)inst32.size();
+      inst32.push_back(y);
+      item_box.clear();
+      item_code.clear();
+      item_box.push_back(fr.nodes[0].bbox);
+      item_code.push_back(flat_inst);
+    }
+  }
+
   sph32.resize(4 * (size_t)ns);
   for (int s = 0; s < ns; s++) {
     const double* c = &sph[4 * (size_t)s];

@@ -44,6 +44,9 @@ RefBvh build_ref_bvh(const std::vector<DAabb>& prim_bounds, size_t leaf_size);
 // Triangle::bounds (geometry.rs:221-225) of a static triangle.
 DAabb triangle_bounds(const double* v9);
 
+// Scenes with at most this many instanced triangles are flattened into world space.
+constexpr int64_t kFlatMaxTris = 1 << 20;
+
 struct HostScene {
   std::string error;
   // ---- copies of the descriptor
@@ -81,7 +84,10 @@ struct HostScene {
   std::vector<int32_t> tlas_items;  // per TLAS leaf slot: item code (kind bit | index)
   std::vector<TriF32> tris32;       // reordered in leaf order
   std::vector<int32_t> tri_orig;    // reordered -> mesh-local original index
-  std::vector<int32_t> tri_mat32;   // reordered -> material
+  std::vector<int32_t> tri_mat32;   // reordered -> material (instance override applied for flattened slots)
+  std::vector<int32_t> tri_inst32;  // flattened slots: instance | TPT_TRI_FLIP (empty when not flattened)
+  bool flat = false;                // instances flattened into one world-space tree
+  int32_t flat_inst = -1;           // index of the identity pseudo-instance that holds that tree
   std::vector<InstF32> inst32;
   std::vector<float> sph32;  // 4 per sphere (FP32 copy, TLAS boxes only)
   std::vector<LightF32> lights32;
@@ -91,6 +97,7 @@ struct HostScene {
   std::vector<int32_t> alias_idx;    // per cell
   std::vector<float> cell_pdf;       // per cell: (w/total) * W * H / (2 pi^2), the pdf numerator
   int max_blas_depth = 0, max_tlas_depth = 0;
+  int flat_depth = 0;  // levels of the flattened tree
 
   bool build(const tpt_scene_desc* d);
   void rebuild_lights();

@@ -19,6 +19,13 @@
 // TLAS item codes: bit 31 set = sphere (index in low bits), clear = instance index.
 #define TPT_ITEM_SPHERE 0x80000000u
 
+// FP32 hit record: `inst` of a hit on a flattened (world-space) triangle carries this bit over
+// the original instance index.
+#define TPT_HIT_FLAT 0x40000000u
+// Flattened triangle's instance word: the instance's A has det < 0, so the world-space
+// e1 x e2 points against the reference's normal.
+#define TPT_TRI_FLIP 0x40000000u
+
 // Hit kind in the FP32 hit record (high bits of `inst`).
 #define TPT_HIT_NONE 0
 #define TPT_HIT_TRI 1

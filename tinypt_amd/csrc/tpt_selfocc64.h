@@ -22,7 +22,7 @@ struct D3 {
 };
 
 // Light index `idx`, shading point p (FP32 promoted), light-sample uniforms (u, v).
-__device__ __noinline__ bool light_self_occluded64(const SceneF64& S, const LightsDev& Ls, int idx, float px, float py,
+__device__ __forceinline__ bool light_self_occluded64(const SceneF64& S, const LightsDev& Ls, int idx, float px, float py,
                                                    float pz, float uf, float vf) {
 #pragma clang fp contract(off)
   auto add = [](D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; };

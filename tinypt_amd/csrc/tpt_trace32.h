@@ -28,17 +28,20 @@ struct SceneF32 {
   const TriF32* __restrict__ tris;
   const int32_t* __restrict__ tri_mat;
   const int32_t* __restrict__ tri_orig;
+  const int32_t* __restrict__ tri_inst;  // flattened slots: original instance | TPT_TRI_FLIP
   const InstF32* __restrict__ inst;
   const int32_t* __restrict__ tlas_items;
   const double* __restrict__ sph;  // c.xyz, r (FP64)
   const int32_t* __restrict__ sph_mat;
   int32_t tlas_root;
   int32_t n_spheres;
+  int32_t flat_inst;  // the pseudo-instance holding the flattened world-space tree, or -1
+  int32_t pad;
 };
 
 struct HitF32 {
   float t;
-  int32_t inst;  // instance index, TPT_ITEM_SPHERE | sphere, or -1 (miss)
+  int32_t inst;  // instance index (| TPT_HIT_FLAT for a flattened triangle), TPT_ITEM_SPHERE | sphere, or -1
   int32_t prim;  // reordered triangle slot, -1 for spheres / miss
 };
 
@@ -207,9 +210,15 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
           if (v < 0.f || u + v > 1.f) continue;
           const float th = dot(E2, qvec) * inv_det;
           if (th < lo || th > best_t) continue;
-          if (th < best_t || cur_key >= best_key) {
-            best_t = th; best_key = cur_key; found = true;
-            hit->t = th; hit->inst = cur_inst; hit->prim = (int32_t)(start + k);
+          int32_t key = cur_key, hinst = cur_inst;
+          if (cur_inst == S.flat_inst) {  // world-space triangle: its own instance decides ties
+            const int32_t i0 = S.tri_inst[start + k] & (int32_t)~TPT_TRI_FLIP;
+            key = TPT_KEY_INST(i0);
+            hinst = (int32_t)((uint32_t)i0 | TPT_HIT_FLAT);
+          }
+          if (th < best_t || key >= best_key) {
+            best_t = th; best_key = key; found = true;
+            hit->t = th; hit->inst = hinst; hit->prim = (int32_t)(start + k);
             if (ANY) return true;
           }
         }

@@ -10,6 +10,8 @@
 namespace tpt {
 
 #define TPT_WF_BLOCK 256
+#define TPT_WF_MAX_BLOCKS 2048
+static_assert(TPT_WF_BLOCK % 64 == 0 && TPT_WF_BLOCK / 64 <= 7, "block_slot keeps per-wave counts in 7 LDS words");
 #define TPT_WF_STACK 24      // LDS stack entries per lane (24 KB per 256-lane block)
 #define TPT_WF_STACK_ADJ 16  // the adjoint kernel keeps 16 in LDS (plus 4 KB of gradient slots)
 
@@ -19,20 +21,23 @@ namespace tpt {
 #define TPT_CNT_ACTIVE 3
 #define TPT_NUM_CNT 8
 
-// Path state, structure of arrays, capacity = paths in flight.
+// Path state, structure of arrays.  The live paths of a bounce sit densely at queue positions
+// 0..n-1 of one of two buffer sets; shading writes the survivors densely into the other set, so
+// every kernel reads its inputs coalesced.  A path carries its id (the slot of its (pixel,
+// sample) in the pass) and gathers radiance in L[id].
 struct PathState {
-  float4* o;     // origin xyz, pdf of the last BSDF sample
-  float4* d;     // direction xyz, flags (bits 0-7 bounce, bit 8 last bounce non-delta)
-  float4* beta;  // throughput
-  float4* L;     // radiance gathered so far
-  uint64_t* rng;
-  uint32_t* pix;
-  int4* hit;       // t bits, instance code, triangle slot
-  float4* sh_o;    // 2 per path: shadow origin, tmax
-  float4* sh_d;    // 2 per path: shadow direction
-  float4* sh_c;    // 2 per path: contribution if unoccluded, w != 0 when present
-  int32_t* q[2];   // ping-pong extension queues
-  int32_t* sq;     // paths with shadow rays this bounce
+  float4* o[2];     // origin xyz, pdf of the last BSDF sample
+  float4* d[2];     // direction xyz, flags (bits 0-7 bounce, bit 8 last bounce non-delta)
+  float4* beta[2];  // throughput xyz, path id (int bits)
+  uint64_t* rng[2];
+  int4* hit;        // closest hit of queue position i: t bits, instance code, triangle slot
+  float4* L;        // radiance gathered so far, by path id
+  // shadow queue, dense by entry: two rays per entry (0: environment, 1: area light)
+  float4* sh_o;   // origin, tmax
+  float4* sh_d;   // direction; [2 j].w = path id (int bits)
+  float4* sh_c;   // contribution if unoccluded, w != 0 when present
+  float4* sh_l;   // light ray: shading point xyz, light-sample u
+  float2* sh_lv;  // light ray: light-sample v, light index (int bits)
   uint32_t* counters;
   uint64_t* stats;  // [0] extension rays, [1] shadow rays, [2] paths
 };
@@ -85,6 +90,7 @@ struct AdaptiveArgs {
 hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s);
 hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shadow(const WfParams& W, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s);
 hipError_t wf_launch_accumulate_adaptive(const WfParams& W, const AdaptiveArgs& A, hipStream_t s);

@@ -75,7 +75,8 @@ def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: in
     truth = torch.from_numpy(prob.truth)
     for it in range(steps):
         t0 = time.perf_counter()
-        img = render_image(prob.dscene, prob.config(prob.spp, seed0 + it), prob.table(x))
+        # independent paths for the adjoint: the L2 gradient is then unbiased
+        img = render_image(prob.dscene, prob.config(prob.spp, seed0 + it), prob.table(x), decorrelate=True)
         loss = ((img - prob.target) ** 2).mean()
         opt.zero_grad()
         loss.backward()
