@@ -221,15 +221,46 @@ int tpt_trace_closest(tpt_ctx* ctx, const tpt_scene* scene, int64_t n, const flo
 
 /* Path-replay adjoint of the FP32 render: given dLoss/dPixel (d_loss_d_pixel[w*h*3], the
  * derivative w.r.t. the resolved pixel acc/acc_w), regenerates every path from its seed and
- * accumulates grad_params[n_materials*4] += sum over paths of dLoss/dparam.  Sampling pdfs
- * are held fixed (detached); BSDF values, MIS weights and emitted radiance are
- * differentiated.  Host pointers. */
+ * accumulates grad_params[n_materials*4] += sum over paths of dLoss/dparam.  Albedo / F0 /
+ * absorption / emission are differentiated in reverse mode along the path (sampling pdfs and
+ * Russian-roulette probabilities held fixed).  GGX roughness (param[3]) takes the central
+ * difference (h = 1e-3) of each path's loss term, replayed with the same random numbers,
+ * because the reference's VNDF sampler (material.rs:327-347) draws from a density whose
+ * support moves with alpha.  No reference counterpart.  Host pointers. */
 int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg,
                        const float* d_loss_d_pixel, double* grad_params, tpt_stats* stats);
 
 /* Update the material table of an uploaded scene in place (mat_param[n_materials*4]); the
  * light CDF is rebuilt when emitter radiance changes.  Used by inverse rendering loops. */
 int tpt_scene_update_materials(tpt_ctx* ctx, tpt_scene* scene, const double* mat_param);
+
+/* ---- scene files ------------------------------------------------------------------------- */
+typedef struct tpt_loaded_scene tpt_loaded_scene;
+
+/* load_scene (mitsuba.rs:137-223): the Mitsuba-XML subset the reference reads -- perspective
+ * sensor, sphere / rectangle / cube / disk / obj shapes with to_world transforms, diffuse /
+ * conductor / roughconductor / dielectric / twosided bsdfs, area emitters, constant and
+ * envmap (Radiance .hdr) scene emitters, and film / sampler / integrator settings, which
+ * update cfg's width, height, spp, max_bounces and rr_start.  Unsupported elements are skipped
+ * with a warning on stderr ("[mitsuba] warning: ..."), as the reference does; the warnings
+ * are also kept with the loaded scene.  Fails on unreadable files, malformed XML or a root
+ * that is not <scene>. */
+int tpt_scene_load_xml(const char* path, tpt_render_config* cfg, tpt_loaded_scene** out);
+
+/* The loaded scene as a descriptor for tpt_scene_upload / tpt_scene_build_host; its arrays
+ * live until tpt_loaded_scene_destroy.  No reference counterpart (the reference returns a
+ * Scene value). */
+int tpt_loaded_scene_desc(const tpt_loaded_scene* scene, const tpt_scene_desc** desc);
+
+/* Newline-separated warnings of the load ("" if none); valid until destroy. */
+const char* tpt_loaded_scene_warnings(const tpt_loaded_scene* scene);
+
+int tpt_loaded_scene_destroy(tpt_loaded_scene* scene);
+
+/* load_obj_triangles (obj_loader.rs:101-109): triangles of a Wavefront OBJ file (v and f
+ * records; polygons fan-triangulated; negative indices count from the end; other records
+ * ignored).  tri_vertices[n*9] (v0, v1, v2); pass NULL to query *n_tris only. */
+int tpt_obj_load(const char* path, int64_t* n_tris, double* tri_vertices);
 
 #ifdef __cplusplus
 }

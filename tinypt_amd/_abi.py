@@ -139,4 +139,9 @@ ENTRY_POINTS = {
     "tpt_scene_build_host": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(_vp)]),
     "tpt_scene_get_lights": (C.c_int, [_vp, _ip, _dp]),
     "tpt_scene_get_env_tables": (C.c_int, [_vp, _dp, _fp, _ip, _fp]),
+    "tpt_scene_load_xml": (C.c_int, [C.c_char_p, C.POINTER(RenderConfigC), C.POINTER(_vp)]),
+    "tpt_loaded_scene_desc": (C.c_int, [_vp, C.POINTER(C.POINTER(SceneDesc))]),
+    "tpt_loaded_scene_warnings": (C.c_char_p, [_vp]),
+    "tpt_loaded_scene_destroy": (C.c_int, [_vp]),
+    "tpt_obj_load": (C.c_int, [C.c_char_p, _lp, _dp]),
 }

@@ -34,9 +34,10 @@ SOURCES = [
     ("k_adjoint.hip", HIP_FLAGS),
     ("k_plain64.hip", HIP_FLAGS + ["-ffp-contract=off"]),
 ]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC]
 HOST_SOURCES = [
-    ("scene_host.cpp", ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math",
-                        "-I", CSRC]),
+    ("scene_host.cpp", HOST_FLAGS),
+    ("scene_xml.cpp", HOST_FLAGS),
 ]
 
 

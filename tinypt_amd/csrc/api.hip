@@ -24,6 +24,14 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+namespace tpt {
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace tpt
+
+namespace {
+
 #define HIPC(x)                                                                                     \
   do {                                                                                              \
     hipError_t e_ = (x);                                                                            \
@@ -191,6 +199,8 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   E.w = H.env_w;
   E.h = H.env_h;
   E.total = H.env_total;
+  E.black = H.env_kind == TPT_ENV_IMAGE &&
+            std::all_of(H.env_rgb.begin(), H.env_rgb.end(), [](double v) { return v == 0.0; });
   if (H.env_kind == TPT_ENV_IMAGE) {
     std::vector<float4> rgb(H.env_rgb32.size() / 4);
     std::memcpy(rgb.data(), H.env_rgb32.data(), sizeof(float) * H.env_rgb32.size());

@@ -126,13 +126,13 @@ __device__ f3 replay(const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 
   f3 T = mk(1.f, 1.f, 1.f), S_upto = mk(0.f, 0.f, 0.f);
   float last_pdf = 0.f;
   bool last_nd = false;
-  const bool env_img = A.env.kind == TPT_ENV_IMAGE;
+  const bool env_img = A.env.kind == TPT_ENV_IMAGE && !A.env.black;  // as the forward kernels
   for (int bounce = 0; bounce < A.max_bounces; bounce++) {
     HitF32 h;
     (*n_ext)++;
     if (!trace_f32<false>(A.S, o, d, (float)TPT_RAY_EPSILON, 1e30f, &h, st)) {
       // miss (integrator.rs:71-83)
-      f3 bg = env_img ? env_lookup32(A.env, d) : sky32(d);
+      f3 bg = A.env.black ? mk(0.f, 0.f, 0.f) : (env_img ? env_lookup32(A.env, d) : sky32(d));
       f3 x0 = had(T, bg);
       float w = 1.f, pe = 0.f;
       if (env_img && last_nd) {

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     if (i < n) {
       if (h4.y == -1) {
         // miss (integrator.rs:71-83)
-        f3 bg = W.env.kind == TPT_ENV_IMAGE ? env_lookup32(W.env, d) : sky32(d);
+        f3 bg = W.env.black ? mk(0.f, 0.f, 0.f) : (W.env.kind == TPT_ENV_IMAGE ? env_lookup32(W.env, d) : sky32(d));
         f3 c = had(beta, bg);
         if (W.env.kind == TPT_ENV_IMAGE && last_nd) c = c * mis_weight(last_pdf, env_pdf32(W.env, d));
         add_radiance(W.ps.L, id, clamp_luminance(c, (float)TPT_FIREFLY_CLAMP));
@@ -161,7 +161,9 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
           } else if (!mat_is_delta(kind)) {
             f3 n_o = dot(ng, d) < 0.f ? ng : -ng;  // oriented_normal (integrator.rs:138-140)
             f3 wo = vnorm(-d);
-            if (W.env.kind == TPT_ENV_IMAGE) {  // nee_environment (integrator.rs:148-172)
+            // nee_environment (integrator.rs:148-172); a black map contributes nothing, so the
+            // FP32 form skips it (its own random streams need not follow the reference's draws)
+            if (W.env.kind == TPT_ENV_IMAGE && !W.env.black) {
               f3 wi, li;
               float pdf_env;
               env_sample32(W.env, rng, &wi, &li, &pdf_env);

@@ -14,8 +14,8 @@ namespace tpt {
 struct EnvDev {
   int32_t kind;  // TPT_ENV_SKY / TPT_ENV_IMAGE
   int32_t w, h;
-  int32_t pad;
-  double total;  // total_weight (env.rs:32)
+  int32_t black;  // image environment whose texels are all zero: it adds nothing, NEE included
+  double total;   // total_weight (env.rs:32)
   // FP32 side
   const float4* __restrict__ rgb32;       // w*h texels
   const float* __restrict__ alias_prob;   // w*h

@@ -17,6 +17,9 @@
 // The traversal keeps the current node in a register and pushes only far children onto a
 // per-lane stack in LDS (stride = block size, so lanes hit distinct banks), overflowing to a
 // per-thread global area when a tree is deeper than the LDS part.
+// Small scenes are flattened at build time: every instance's triangles in world space under one
+// tree, held by an identity pseudo-instance (SceneF32::flat_inst).  A hit there reports the
+// triangle's own instance (TPT_HIT_FLAT | instance) and takes that instance's tie key.
 #pragma once
 
 #include "tpt_math.h"
