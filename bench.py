@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-rmse", action="store_true", help="skip the 2048-spp FP64 reference image")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU render")
+    ap.add_argument("--no-inverse", action="store_true", help="skip the inverse-rendering gradient loop")
+    ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -151,6 +153,21 @@ def main():
         r = oracle.OracleScene(scene).render(64, 64, 16, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED)
         bytes_per_ray = algorithmic_bytes_per_ray(r["counters"])
 
+    inverse = None
+    if not args.no_inverse:
+        # Cornell box + 16 GGX spheres (64 parameters), 256x256, 16 spp forward, depth 8, target at
+        # 1024 spp; one step = forward render + path-replay adjoint + Adam update
+        from tinypt_amd.inverse import InverseProblem, optimize
+        prob = InverseProblem(ctx=ds.ctx)
+        hist = optimize(prob, steps=args.inverse_steps, lr=0.02)
+        inverse = {"s_per_iter": hist["s_per_iter"], "steps": args.inverse_steps, "loss_first": hist["loss"][0],
+                   "loss_last10_mean": float(np.mean(hist["loss"][-10:])),
+                   "param_error_init": hist["init_error"], "param_error_final": hist["param_error"][-1],
+                   "f0_error_init": hist["init_f0_error"], "f0_error_final": hist["f0_error"][-1],
+                   "alpha_error_init": hist["init_alpha_error"], "alpha_error_final": hist["alpha_error"][-1],
+                   "config": "cornell + 16 GGX spheres, 256x256, 16 spp, depth 8, target 1024 spp, Adam lr 0.02"}
+        log(f"inverse: {inverse}")
+
     avg_launch_ms = trace_ms / max(trace_launches, 1)
     rays_per_launch = trace_rays / max(trace_launches, 1)
     achieved = bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
@@ -167,7 +184,8 @@ def main():
                        "accumulation": "fp64"},
             "mrays_per_s": mrays, "fp64_plain_paths_per_s": plain_rate,
             "speedup_vs_fp64_plain": value / plain_rate if plain_rate else None,
-            "rmse": rmse, "roofline": roofline, "cpu_baseline": cpu}
+            "rmse": rmse, "grad_s_per_iter": inverse["s_per_iter"] if inverse else None, "inverse": inverse,
+            "roofline": roofline, "cpu_baseline": cpu}
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
     print(json.dumps(line), flush=True)

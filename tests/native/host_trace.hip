@@ -14,7 +14,7 @@ extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const flo
   HostScene H;
   if (!H.build(d)) return -1;
   SceneF32 S{};
-  S.nodes = H.nodes2.data();
+  S.nodes = H.nodes4.data();
   S.tris = H.tris32.data();
   S.tri_mat = H.tri_mat32.data();
   S.tri_orig = H.tri_orig.data();

@@ -2,8 +2,8 @@
 
 Each translation unit is compiled separately so that the plain FP64 kernel and the host scene
 build get -ffp-contract=off (their f64 arithmetic restates the reference's, which never fuses
-into FMA), while the FP32 kernels keep contraction.  Packed FP32 instructions are disabled
-for all device code.
+into FMA), while the FP32 kernels contract within expressions.  Packed FP32 instructions are
+disabled for all device code.
 
     python -m tinypt_amd.build [-v]
 """
@@ -27,11 +27,14 @@ GXX = shutil.which("g++") or "g++"
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC]
+# FP32 kernels: contraction only inside one source expression (-ffp-contract=on), so that the
+# forward kernels and the adjoint's replay of the same code round identically (a path replayed
+# for the gradient is then the path the image summed).
 SOURCES = [
     ("api.hip", HIP_FLAGS),
-    ("k_trace.hip", HIP_FLAGS),
-    ("k_wavefront.hip", HIP_FLAGS),
-    ("k_adjoint.hip", HIP_FLAGS),
+    ("k_trace.hip", HIP_FLAGS + ["-ffp-contract=on"]),
+    ("k_wavefront.hip", HIP_FLAGS + ["-ffp-contract=on"]),
+    ("k_adjoint.hip", HIP_FLAGS + ["-ffp-contract=on"]),
     ("k_plain64.hip", HIP_FLAGS + ["-ffp-contract=off"]),
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC]

@@ -74,7 +74,8 @@ struct tpt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   // wavefront scratch (grow-only)
-  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o, sh_d, sh_c, sh_l, sh_lv, counters, stats, gstack;
+  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o[2], sh_d[2], sh_c[2], sh_l, sh_lv, counters, stats,
+      gstack;
   // adaptive / plain per-pixel state
   DevBuf act0, act1, nsamp, welford, rngpix, done;
   size_t ps_cap = 0;
@@ -83,7 +84,7 @@ struct tpt_ctx {
 struct tpt_scene {
   tpt_ctx* ctx = nullptr;  // null for a host-only scene (tpt_scene_build_host)
   HostScene host;
-  DevBuf nodes2, tris32, tri_mat32, tri_orig, tri_inst, inst32, tlas_items, sph64, sph_mat;
+  DevBuf nodes4, tris32, tri_mat32, tri_orig, tri_inst, inst32, tlas_items, sph64, sph_mat;
   DevBuf ref_nodes, ref_indices, tri_v, tri_mat, meshes, inst64;
   DevBuf mat_kind, mat_p32, mat_p64;
   DevBuf l32, cdf32, l64, cdf64;
@@ -174,7 +175,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   }
   hipStream_t s = ctx->stream;
   std::vector<double> sph64 = H.sph;
-  HIPC(upload(sc->nodes2, H.nodes2, s));
+  HIPC(upload(sc->nodes4, H.nodes4, s));
   HIPC(upload(sc->tris32, H.tris32, s));
   HIPC(upload(sc->tri_mat32, H.tri_mat32, s));
   HIPC(upload(sc->tri_orig, H.tri_orig, s));
@@ -221,7 +222,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   }
   // FP32 view
   SceneF32& S = sc->s32;
-  S.nodes = sc->nodes2.as<Node2>();
+  S.nodes = sc->nodes4.as<Node4>();
   S.tris = sc->tris32.as<TriF32>();
   S.tri_mat = sc->tri_mat32.as<int32_t>();
   S.tri_orig = sc->tri_orig.as<int32_t>();
@@ -259,7 +260,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   C.lens_radius = (float)H.cam.lens_radius;
   C.focus_dist = (float)H.cam.focus_dist;
   HIPC(hipStreamSynchronize(s));
-  for (DevBuf* b : {&sc->nodes2, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->tri_inst, &sc->inst32, &sc->tlas_items, &sc->sph64,
+  for (DevBuf* b : {&sc->nodes4, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->tri_inst, &sc->inst32, &sc->tlas_items, &sc->sph64,
                     &sc->sph_mat, &sc->ref_nodes, &sc->ref_indices, &sc->tri_v, &sc->tri_mat, &sc->meshes, &sc->inst64,
                     &sc->mat_kind, &sc->mat_p32, &sc->mat_p64, &sc->l32, &sc->cdf32, &sc->l64, &sc->cdf64, &sc->env_rgb32,
                     &sc->alias_prob, &sc->alias_idx, &sc->cell_pdf, &sc->env_rgb64, &sc->row_cdf, &sc->col_cdf})
@@ -408,9 +409,11 @@ int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
     }
     HIPC(c->ps_hit.ensure(P * 16));
     HIPC(c->ps_L.ensure(P * 16));
-    HIPC(c->sh_o.ensure(P * 32));
-    HIPC(c->sh_d.ensure(P * 32));
-    HIPC(c->sh_c.ensure(P * 32));
+    for (int q = 0; q < 2; q++) {
+      HIPC(c->sh_o[q].ensure(P * 16));
+      HIPC(c->sh_d[q].ensure(P * 16));
+      HIPC(c->sh_c[q].ensure(P * 16));
+    }
     HIPC(c->sh_l.ensure(P * 16));
     HIPC(c->sh_lv.ensure(P * 8));
     c->ps_cap = P;
@@ -438,9 +441,11 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   }
   ps.hit = c->ps_hit.as<int4>();
   ps.L = c->ps_L.as<float4>();
-  ps.sh_o = c->sh_o.as<float4>();
-  ps.sh_d = c->sh_d.as<float4>();
-  ps.sh_c = c->sh_c.as<float4>();
+  for (int q = 0; q < 2; q++) {
+    ps.sh_o[q] = c->sh_o[q].as<float4>();
+    ps.sh_d[q] = c->sh_d[q].as<float4>();
+    ps.sh_c[q] = c->sh_c[q].as<float4>();
+  }
   ps.sh_l = c->sh_l.as<float4>();
   ps.sh_lv = c->sh_lv.as<float2>();
   ps.counters = c->counters.as<uint32_t>();
@@ -493,8 +498,11 @@ int wf_bounces(tpt_ctx* c, const WfParams& W, uint32_t P, EventTimer& tt, uint64
     HIPC(tt.mark());
     (*launches)++;
     HIPC(wf_launch_shade(W, cur, P, s));
-    if (W.lights.n > 0) HIPC(wf_launch_light_selfocc(W, P, s));
-    HIPC(wf_launch_shadow(W, P, s));
+    if (W.lights.n > 0) {
+      HIPC(wf_launch_light_selfocc(W, P, s));
+      HIPC(wf_launch_shadow(W, 1, P, s));
+    }
+    if (W.env.kind == TPT_ENV_IMAGE && !W.env.black) HIPC(wf_launch_shadow(W, 0, P, s));
     cur ^= 1;
   }
   return TPT_OK;

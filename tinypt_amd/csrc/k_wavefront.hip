@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
     W.ps.counters[TPT_CNT_Q0] = P;
     W.ps.counters[TPT_CNT_Q1] = 0;
     W.ps.counters[TPT_CNT_SH] = 0;
+    W.ps.counters[TPT_CNT_SHL] = 0;
   }
 }
 
@@ -82,6 +83,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_extend(WfParams W, int cur) {
   if (gtid() == 0) {
     W.ps.counters[cur ? TPT_CNT_Q0 : TPT_CNT_Q1] = 0;  // next queue
     W.ps.counters[TPT_CNT_SH] = 0;
+    W.ps.counters[TPT_CNT_SHL] = 0;
     atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
   }
   for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
@@ -102,11 +104,12 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
   const uint32_t n = W.ps.counters[cur ? TPT_CNT_Q1 : TPT_CNT_Q0];
   const int nxt = cur ^ 1;
   uint32_t* cout = &W.ps.counters[cur ? TPT_CNT_Q0 : TPT_CNT_Q1];
-  uint32_t* csh = &W.ps.counters[TPT_CNT_SH];
+  uint32_t* csh_env = &W.ps.counters[TPT_CNT_SH];
+  uint32_t* csh_light = &W.ps.counters[TPT_CNT_SHL];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
-    bool alive = false, has_shadow = false;
+    bool alive = false, has_env = false, has_light = false;
     // path state (valid when i < n)
     f3 o = mk(0.f, 0.f, 0.f), d = o, beta = o;
     float last_pdf = 0.f;
@@ -129,7 +132,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     const bool last_nd = (flags & 0x100u) != 0;
     // ---- phase 1: miss, Russian roulette, emission, next-event estimation
     float4 sh_c0 = make_float4(0.f, 0.f, 0.f, 0.f), sh_c1 = sh_c0;
-    float4 sh_o0 = sh_c0, sh_d0 = sh_c0, sh_o1 = sh_c0, sh_d1 = sh_c0, sh_l = sh_c0;
+    float4 sh_o0 = sh_c0, sh_d0 = sh_c0, sh_o1 = sh_c0, sh_d1 = sh_c0, sh_l = sh_c0;  // 0: environment, 1: light
     float2 sh_lv = make_float2(0.f, 0.f);
     bool go = false;  // hit a scattering surface: sample the BSDF in phase 2
     f3 p = mk(0.f, 0.f, 0.f), ng = p;
@@ -176,9 +179,9 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
                 if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
                   f3 so = p + (float)TPT_RAY_EPSILON * wi;
                   sh_o0 = make_float4(so.x, so.y, so.z, 1e30f);
-                  sh_d0 = make_float4(wi.x, wi.y, wi.z, 0.f);
+                  sh_d0 = make_float4(wi.x, wi.y, wi.z, __int_as_float(id));
                   sh_c0 = make_float4(c.x, c.y, c.z, 1.f);
-                  has_shadow = true;
+                  has_env = true;
                 }
               }
             }
@@ -197,11 +200,11 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
                   if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
                     f3 so = p + (float)TPT_RAY_EPSILON * wi;
                     sh_o1 = make_float4(so.x, so.y, so.z, light_shadow_tmax32(dist));
-                    sh_d1 = make_float4(wi.x, wi.y, wi.z, 0.f);
+                    sh_d1 = make_float4(wi.x, wi.y, wi.z, __int_as_float(id));
                     sh_c1 = make_float4(c.x, c.y, c.z, 1.f);
                     sh_l = make_float4(p.x, p.y, p.z, ls.u);  // for k_light_selfocc
                     sh_lv = make_float2(ls.v, __int_as_float(ls.idx));
-                    has_shadow = true;
+                    has_light = true;
                   }
                 }
               }
@@ -210,17 +213,19 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
         }
       }
     }
-    const uint32_t j = block_slot(has_shadow, csh, s_slot);
-    if (has_shadow) {
-      sh_d0.w = __int_as_float(id);
-      W.ps.sh_o[2 * j] = sh_o0;
-      W.ps.sh_d[2 * j] = sh_d0;
-      W.ps.sh_c[2 * j] = sh_c0;
-      W.ps.sh_o[2 * j + 1] = sh_o1;
-      W.ps.sh_d[2 * j + 1] = sh_d1;
-      W.ps.sh_c[2 * j + 1] = sh_c1;
-      W.ps.sh_l[j] = sh_l;
-      W.ps.sh_lv[j] = sh_lv;
+    const uint32_t je = block_slot(has_env, csh_env, s_slot);
+    if (has_env) {
+      W.ps.sh_o[0][je] = sh_o0;
+      W.ps.sh_d[0][je] = sh_d0;
+      W.ps.sh_c[0][je] = sh_c0;
+    }
+    const uint32_t jl = block_slot(has_light, csh_light, s_slot);
+    if (has_light) {
+      W.ps.sh_o[1][jl] = sh_o1;
+      W.ps.sh_d[1][jl] = sh_d1;
+      W.ps.sh_c[1][jl] = sh_c1;
+      W.ps.sh_l[jl] = sh_l;
+      W.ps.sh_lv[jl] = sh_lv;
     }
     // ---- phase 2: BSDF sampling; survivors go densely into the next buffer set
     bool bsdf_nd = false;
@@ -250,37 +255,35 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
 // in FP64 (tpt_selfocc64.h) here, in a kernel of its own so the FP64 registers do not weigh on
 // k_shade.  A self-occluded ray is dropped before k_shadow traces it.
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
-  const uint32_t n = W.ps.counters[TPT_CNT_SH];
+  const uint32_t n = W.ps.counters[TPT_CNT_SHL];
   for (uint32_t j = gtid(); j < n; j += gridDim.x * blockDim.x) {
-    float4 c = W.ps.sh_c[2 * j + 1];
-    if (c.w == 0.f) continue;
     float4 l = W.ps.sh_l[j];
     float2 lv = W.ps.sh_lv[j];
     if (light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x))
-      W.ps.sh_c[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+      W.ps.sh_c[1][j] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
-__global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W) {
+// Any hit for one shadow queue (0: environment, 1: area light); an unoccluded ray adds its
+// contribution to its path's radiance.  A path has at most one ray per queue, so the
+// read-modify-write of L[id] has a single writer per launch.
+__global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
-  const uint32_t n = W.ps.counters[TPT_CNT_SH];
+  const uint32_t n = W.ps.counters[queue ? TPT_CNT_SHL : TPT_CNT_SH];
+  const float4* __restrict__ qo = W.ps.sh_o[queue];
+  const float4* __restrict__ qd = W.ps.sh_d[queue];
+  const float4* __restrict__ qc = W.ps.sh_c[queue];
   LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
                TPT_WF_STACK};
   uint32_t traced = 0;
   for (uint32_t j = gtid(); j < n; j += gridDim.x * blockDim.x) {
-    f3 add = mk(0.f, 0.f, 0.f);
-    int32_t id = 0;
-    for (int k = 0; k < 2; k++) {
-      float4 d4 = W.ps.sh_d[2 * j + k];
-      if (k == 0) id = __float_as_int(d4.w);
-      float4 c = W.ps.sh_c[2 * j + k];
-      if (c.w == 0.f) continue;
-      float4 o4 = W.ps.sh_o[2 * j + k];
-      HitF32 h;
-      traced++;
-      if (!trace_f32<true>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, o4.w, &h, st)) add = add + f3of(c);
-    }
-    if (add.x != 0.f || add.y != 0.f || add.z != 0.f) add_radiance(W.ps.L, id, add);
+    const float4 c = qc[j];
+    if (c.w == 0.f) continue;  // dropped (self-occluded light sample)
+    const float4 o4 = qo[j], d4 = qd[j];
+    HitF32 h;
+    traced++;
+    if (!trace_f32<true>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, o4.w, &h, st))
+      add_radiance(W.ps.L, __float_as_int(d4.w), f3of(c));
   }
   // one atomic per wave for the shadow-ray count
   uint32_t tot = traced;
@@ -383,8 +386,8 @@ hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_
   hipLaunchKernelGGL(k_light_selfocc, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W);
   return hipGetLastError();
 }
-hipError_t wf_launch_shadow(const WfParams& W, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W);
+hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, queue);
   return hipGetLastError();
 }
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s) {

@@ -195,6 +195,64 @@ static int32_t to_node2(const RefBvh& r, int32_t id, int64_t slot_base, std::vec
   return me;
 }
 
+// ------------------------------------------------------------------ four-wide collapse (own design)
+// A Node2 subtree becomes one Node4 by opening, while fewer than four children are held, the
+// inner child with the largest box area; leaves and empty refs pass through unchanged.
+static int32_t collapse4(const std::vector<Node2>& n2, int32_t ref, std::vector<Node4>& out) {
+  if (ref < 0 || ref == TPT_REF_EMPTY) return ref;
+  struct Kid {
+    int32_t ref;
+    float b[6];
+  };
+  auto area = [](const float* b) {
+    float x = b[3] - b[0], y = b[4] - b[1], z = b[5] - b[2];
+    return x * y + y * z + z * x;
+  };
+  std::vector<Kid> kids;
+  const Node2& root = n2[(size_t)ref];
+  for (int c = 0; c < 2; c++) {
+    Kid k;
+    k.ref = root.child[c];
+    std::copy(root.b + 6 * c, root.b + 6 * c + 6, k.b);
+    kids.push_back(k);
+  }
+  while (kids.size() < 4) {
+    int best = -1;
+    float best_a = -1.f;
+    for (size_t i = 0; i < kids.size(); i++)
+      if (kids[i].ref >= 0 && kids[i].ref != TPT_REF_EMPTY && area(kids[i].b) > best_a) {
+        best = (int)i;
+        best_a = area(kids[i].b);
+      }
+    if (best < 0) break;
+    const Node2& n = n2[(size_t)kids[(size_t)best].ref];
+    Kid a, b;
+    a.ref = n.child[0];
+    std::copy(n.b, n.b + 6, a.b);
+    b.ref = n.child[1];
+    std::copy(n.b + 6, n.b + 12, b.b);
+    kids[(size_t)best] = a;
+    kids.insert(kids.begin() + best + 1, b);
+  }
+  const int32_t me = (int32_t)out.size();
+  out.push_back(Node4{});
+  Node4 tmp{};
+  for (int i = 0; i < 4; i++) {
+    if (i < (int)kids.size()) {
+      const Kid& k = kids[(size_t)i];
+      tmp.lo_x[i] = k.b[0]; tmp.lo_y[i] = k.b[1]; tmp.lo_z[i] = k.b[2];
+      tmp.hi_x[i] = k.b[3]; tmp.hi_y[i] = k.b[4]; tmp.hi_z[i] = k.b[5];
+      tmp.child[i] = collapse4(n2, k.ref, out);
+    } else {
+      tmp.lo_x[i] = tmp.lo_y[i] = tmp.lo_z[i] = 1.f;  // never tested: the traversal skips empty slots
+      tmp.hi_x[i] = tmp.hi_y[i] = tmp.hi_z[i] = -1.f;
+      tmp.child[i] = TPT_REF_EMPTY;
+    }
+  }
+  out[(size_t)me] = tmp;
+  return me;
+}
+
 // ------------------------------------------------------------------ TLAS (own design)
 // Full-sweep SAH over item boxes, one item per leaf.  The reference has no top-level tree; it
 // visits every instance and then every sphere for each ray (world.rs:86-118).
@@ -550,6 +608,16 @@ bool HostScene::build(const tpt_scene_desc* d) {
     for (size_t k = 0; k < tb.order.size(); k++) tlas_items[k] = item_code[(size_t)tb.order[k]];
   }
   if (nodes2.size() > (size_t)INT32_MAX / 2) { error = "too many nodes"; return false; }
+  // ---- four-wide form of every tree the FP32 kernels enter
+  nodes4.clear();
+  nodes4.reserve(nodes2.size() / 2 + 16);
+  for (int m = 0; m < nmesh; m++) mesh_root2[(size_t)m] = collapse4(nodes2, mesh_root2[(size_t)m], nodes4);
+  for (int m = 0; m < nmesh; m++) meshes[(size_t)m].pad = mesh_root2[(size_t)m];
+  for (InstF32& y : inst32) {
+    if (y.mesh >= 0) y.root = mesh_root2[(size_t)y.mesh];
+  }
+  if (flat) inst32[(size_t)flat_inst].root = collapse4(nodes2, inst32[(size_t)flat_inst].root, nodes4);
+  tlas_root = collapse4(nodes2, tlas_root, nodes4);
 
   // ---- environment: EnvMap::from_pixels (env.rs:61-89) + alias table over the same weights
   if (env_kind == TPT_ENV_IMAGE) {

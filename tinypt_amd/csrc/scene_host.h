@@ -78,7 +78,8 @@ struct HostScene {
   double env_total = 0.0;
 
   // ---- derived, FP32 (wavefront kernels)
-  std::vector<Node2> nodes2;      // BLAS nodes of all meshes, then TLAS nodes
+  std::vector<Node2> nodes2;      // BLAS nodes of all meshes, then TLAS nodes (binary form)
+  std::vector<Node4> nodes4;      // the same trees collapsed four-wide; roots below index these
   std::vector<int32_t> mesh_root2;  // per mesh: child ref of its root
   int32_t tlas_root = TPT_REF_EMPTY;
   std::vector<int32_t> tlas_items;  // per TLAS leaf slot: item code (kind bit | index)

@@ -39,6 +39,15 @@ struct alignas(16) Node2 {
 };
 static_assert(sizeof(Node2) == 64, "Node2 is one 64 B fetch");
 
+// Four-wide node, the form the FP32 kernels traverse: each binary Node2 subtree is collapsed so
+// that one 128 B fetch tests up to four child boxes (x, y, z bounds as float4 over the children).
+struct alignas(16) Node4 {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  int32_t child[4];  // Node2 reference encoding; TPT_REF_EMPTY for an unused slot
+  int32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is one 128 B fetch");
+
 struct alignas(16) TriF32 {
   float v0[4];  // xyz, w unused
   float e1[4];  // v1 - v0 (computed in FP64, rounded)

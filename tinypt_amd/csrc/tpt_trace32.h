@@ -27,7 +27,7 @@
 namespace tpt {
 
 struct SceneF32 {
-  const Node2* __restrict__ nodes;
+  const Node4* __restrict__ nodes;
   const TriF32* __restrict__ tris;
   const int32_t* __restrict__ tri_mat;
   const int32_t* __restrict__ tri_orig;
@@ -119,36 +119,46 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
   int sp = 0;
   bool found = false;
   for (;;) {
-    // ---- descend inner nodes: both children's boxes from one 64 B node
+    // ---- descend inner nodes: four child boxes from one 128 B node; the nearest hit child
+    // is entered, the others are pushed farthest first
     bool have = true;
     while (ref >= 0 && ref != TPT_STACK_SENTINEL) {
       const float4* p = reinterpret_cast<const float4*>(S.nodes + ref);
-      const float4 a = p[0], b = p[1], c = p[2];
-      const int4 m = reinterpret_cast<const int4*>(p)[3];
-      float x0 = fmaf(a.x, inv.x, -oi.x), x1 = fmaf(a.w, inv.x, -oi.x);
-      float y0 = fmaf(a.y, inv.y, -oi.y), y1 = fmaf(b.x, inv.y, -oi.y);
-      float z0 = fmaf(a.z, inv.z, -oi.z), z1 = fmaf(b.y, inv.z, -oi.z);
-      const float ln = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
-      const float lf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
-      x0 = fmaf(b.z, inv.x, -oi.x); x1 = fmaf(c.y, inv.x, -oi.x);
-      y0 = fmaf(b.w, inv.y, -oi.y); y1 = fmaf(c.z, inv.y, -oi.y);
-      z0 = fmaf(c.x, inv.z, -oi.z); z1 = fmaf(c.w, inv.z, -oi.z);
-      const float rn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
-      const float rf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
-      const bool hl = ln <= lf, hr = rn <= rf;
-      if (hl && hr) {
-        // nearer first; the reference pushes far then near with `a_t0 <= b_t0` choosing left
-        const bool left_first = ln <= rn;
-        st.push(sp, left_first ? m.y : m.x);
-        ref = left_first ? m.x : m.y;
-      } else if (hl) {
-        ref = m.x;
-      } else if (hr) {
-        ref = m.y;
-      } else {
+      const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+      const int4 ch = reinterpret_cast<const int4*>(p)[6];
+      float dn[4];
+      int32_t cr[4] = {ch.x, ch.y, ch.z, ch.w};
+      const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
+      int nhit = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float x0 = fmaf(blx[k], inv.x, -oi.x), x1 = fmaf(bhx[k], inv.x, -oi.x);
+        const float y0 = fmaf(bly[k], inv.y, -oi.y), y1 = fmaf(bhy[k], inv.y, -oi.y);
+        const float z0 = fmaf(blz[k], inv.z, -oi.z), z1 = fmaf(bhz[k], inv.z, -oi.z);
+        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
+        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
+        const bool hit = tn <= tf && cr[k] != TPT_REF_EMPTY;
+        dn[k] = hit ? tn : INFINITY;
+        nhit += hit ? 1 : 0;
+      }
+      if (nhit == 0) {
         have = false;
         break;
       }
+      // sort (distance, child) ascending: 5-comparator network; misses sort last
+#define TPT_CSWAP(a, b)                                              \
+  if (dn[b] < dn[a]) {                                               \
+    const float td = dn[a]; dn[a] = dn[b]; dn[b] = td;                \
+    const int32_t tr = cr[a]; cr[a] = cr[b]; cr[b] = tr;              \
+  }
+      TPT_CSWAP(0, 1) TPT_CSWAP(2, 3) TPT_CSWAP(0, 2) TPT_CSWAP(1, 3) TPT_CSWAP(1, 2)
+#undef TPT_CSWAP
+      if (nhit > 3) st.push(sp, cr[3]);
+      if (nhit > 2) st.push(sp, cr[2]);
+      if (nhit > 1) st.push(sp, cr[1]);
+      ref = cr[0];
     }
     if (have) {
       if (ref == TPT_STACK_SENTINEL) {
@@ -172,6 +182,13 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
         } else {
           const InstF32& I = S.inst[item];
           const int32_t root = I.root;
+          if (root != TPT_REF_EMPTY && item == S.flat_inst) {
+            // the flattened world-space tree: the ray stays in world space
+            cur_inst = item;
+            st.push(sp, TPT_STACK_SENTINEL);
+            ref = root;
+            continue;
+          }
           if (root != TPT_REF_EMPTY) {
             const float4* w = reinterpret_cast<const float4*>(I.w2o);
             const float4 r0 = w[0], r1 = w[1], r2 = w[2];

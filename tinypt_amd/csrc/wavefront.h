@@ -17,8 +17,9 @@ static_assert(TPT_WF_BLOCK % 64 == 0 && TPT_WF_BLOCK / 64 <= 7, "block_slot keep
 
 #define TPT_CNT_Q0 0
 #define TPT_CNT_Q1 1
-#define TPT_CNT_SH 2
+#define TPT_CNT_SH 2  // environment shadow queue
 #define TPT_CNT_ACTIVE 3
+#define TPT_CNT_SHL 4  // area-light shadow queue
 #define TPT_NUM_CNT 8
 
 // Path state, structure of arrays.  The live paths of a bounce sit densely at queue positions
@@ -32,12 +33,12 @@ struct PathState {
   uint64_t* rng[2];
   int4* hit;        // closest hit of queue position i: t bits, instance code, triangle slot
   float4* L;        // radiance gathered so far, by path id
-  // shadow queue, dense by entry: two rays per entry (0: environment, 1: area light)
-  float4* sh_o;   // origin, tmax
-  float4* sh_d;   // direction; [2 j].w = path id (int bits)
-  float4* sh_c;   // contribution if unoccluded, w != 0 when present
-  float4* sh_l;   // light ray: shading point xyz, light-sample u
-  float2* sh_lv;  // light ray: light-sample v, light index (int bits)
+  // shadow rays, one dense queue per kind: [0] environment, [1] area light
+  float4* sh_o[2];  // origin, tmax
+  float4* sh_d[2];  // direction, path id (int bits)
+  float4* sh_c[2];  // contribution if unoccluded; w = 1, set to 0 when the ray is dropped
+  float4* sh_l;     // light queue: shading point xyz, light-sample u
+  float2* sh_lv;    // light queue: light-sample v, light index (int bits)
   uint32_t* counters;
   uint64_t* stats;  // [0] extension rays, [1] shadow rays, [2] paths
 };
@@ -91,7 +92,7 @@ hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t 
 hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s);
-hipError_t wf_launch_shadow(const WfParams& W, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s);
 hipError_t wf_launch_accumulate_adaptive(const WfParams& W, const AdaptiveArgs& A, hipStream_t s);
 uint32_t wf_grid_threads(uint32_t n);
