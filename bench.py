@@ -64,6 +64,25 @@ def cpu_baseline(scene, seconds_target=15.0):
             "mrays_per_s": (c["rays_ext"] + c["rays_shadow"]) / r["seconds"] / 1e6}, c
 
 
+def hip_copy_calibration(nbytes):
+    """One device-to-device copy of `nbytes` through the HIP runtime's copy kernel: a kernel of
+    known traffic, against which the FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,13 +129,8 @@ def main():
     log(f"fp32 wavefront: {secs * 1e3:.1f} ms/step, {value / 1e6:.1f} M paths/s, {mrays:.1f} Mrays/s")
 
     if args.calibrate_copy:
-        import torch
-        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("calibration copy: 1 GiB read + 1 GiB written")
-        del a, b
+        hip_copy_calibration(1 << 30)
+        log("calibration copy: 1 GiB device-to-device (1 GiB read + 1 GiB written)")
 
     # plain FP64 GPU form on the same workload
     plain_rate = None

@@ -33,6 +33,9 @@ class InverseProblem:
     spp: int = 16
     max_bounces: int = 8
     rr_start: int = 3
+    radius: float = 0.12
+    with_cubes: bool = True
+    learn_alpha: bool = True
     ctx: object = None
     scene: object = field(init=False)
     dscene: DeviceScene = field(init=False)
@@ -46,9 +49,11 @@ class InverseProblem:
     def __post_init__(self):
         self.truth = inverse_params(self.truth_seed, self.n_spheres)
         self.init = inverse_params(self.init_seed, self.n_spheres)
-        self.scene = inverse_scene(self.truth, self.width, self.height, self.n_spheres)
+        if not self.learn_alpha:  # roughness known: start from the true values
+            self.init[:, 3] = self.truth[:, 3]
+        self.scene = inverse_scene(self.truth, self.width, self.height, self.n_spheres, self.radius, self.with_cubes)
         self.dscene = DeviceScene(self.scene, self.ctx)
-        self.mat_ids = inverse_material_ids(self.n_spheres)
+        self.mat_ids = inverse_material_ids(self.n_spheres, self.with_cubes)
         self.base = np.array([m.param for m in self.scene.mats], dtype=np.float64)
         t0 = time.perf_counter()
         out = render(self.dscene, self.config(self.target_spp, seed=0x7A26E7), want_sq=False)
@@ -80,11 +85,13 @@ def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: in
         loss = ((img - prob.target) ** 2).mean()
         opt.zero_grad()
         loss.backward()
+        if not prob.learn_alpha:
+            x.grad[:, 3] = 0.0
         opt.step()
         with torch.no_grad():
             x.clamp_(0.01, 1.0)
         times.append(time.perf_counter() - t0)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
         err = (x.detach() - truth).abs()
         errs.append(float(err.mean()))
         f0_errs.append(float(err[:, :3].mean()))

@@ -308,18 +308,25 @@ def instanced_scene(width: int = 1920, height: int = 1080, n_variants: int = 4, 
 
 
 # ---------------------------------------------------------------- inverse rendering
-def inverse_scene(params: np.ndarray, width: int = 256, height: int = 256, n_spheres: int = 16) -> Scene:
-    """Cornell box plus ``n_spheres`` GGX spheres; params (n_spheres, 4) = F0 rgb, alpha."""
+def inverse_scene(params: np.ndarray, width: int = 256, height: int = 256, n_spheres: int = 16, radius: float = 0.12,
+                  with_cubes: bool = True) -> Scene:
+    """Cornell box plus ``n_spheres`` GGX spheres on a grid; params (n_spheres, 4) = F0 rgb,
+    alpha.  The material of sphere i is ``inverse_material_ids(n_spheres, with_cubes)[i]``."""
     params = np.asarray(params, dtype=np.float64).reshape(n_spheres, 4)
     b = SceneBuilder()
     cornell_geometry(b)
+    if not with_cubes:  # drop the two boxes (shapes 5 and 6), keep walls and light
+        del b.world.meshes[5:7], b.world.instances[5:7], b.mats[5:7]
+        for k, inst in enumerate(b.world.instances):
+            inst.mesh_id = k
+            b.world.meshes[k].mat_ids[:] = k
     k = int(math.ceil(math.sqrt(n_spheres)))
     for i in range(n_spheres):
         gx, gz = i % k, i // k
         x = -0.75 + 1.5 * (gx + 0.5) / k
         z = -0.75 + 1.5 * (gz + 0.5) / k
-        y = 0.13 + 0.45 * ((i * 7) % 5) / 4.0
-        b.sphere((x, y, z), 0.12, Ggx(tuple(params[i, :3]), float(params[i, 3])))
+        y = 0.13 + 0.45 * ((i * 7) % 5) / 4.0 if with_cubes else radius + 0.01
+        b.sphere((x, y, z), radius, Ggx(tuple(params[i, :3]), float(params[i, 3])))
     return b.build(cornell_camera(width, height), EnvMap.constant((0.0, 0.0, 0.0)))
 
 
@@ -328,6 +335,7 @@ def inverse_params(seed: int = 5, n_spheres: int = 16) -> np.ndarray:
     return np.random.default_rng(seed).uniform(0.2, 0.8, (n_spheres, 4))
 
 
-def inverse_material_ids(n_spheres: int = 16) -> List[int]:
-    """Material index of each sphere in ``inverse_scene`` (8 Cornell shapes come first)."""
-    return [8 + i for i in range(n_spheres)]
+def inverse_material_ids(n_spheres: int = 16, with_cubes: bool = True) -> List[int]:
+    """Material index of each sphere in ``inverse_scene`` (the Cornell shapes come first)."""
+    first = 8 if with_cubes else 6
+    return [first + i for i in range(n_spheres)]
