@@ -64,6 +64,22 @@ def cpu_baseline(scene, seconds_target=15.0):
             "mrays_per_s": (c["rays_ext"] + c["rays_shadow"]) / r["seconds"] / 1e6}, c
 
 
+TRAFFIC_PROFILE = "profiles/wavefront_cornell_512x512_256spp_hbm_counters.md"
+
+
+def measured_traffic():
+    """HBM bytes per k_extend launch on this workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, each
+    scaled by the counter's reading for a 1 GiB copy (the two counter runs and the calibration
+    are recorded in TRAFFIC_PROFILE, whose last line holds the figure)."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("k_extend_traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def hip_copy_calibration(nbytes):
     """One device-to-device copy of `nbytes` through the HIP runtime's copy kernel: a kernel of
     known traffic, against which the FETCH_SIZE / WRITE_SIZE counters are scaled."""
@@ -185,10 +201,13 @@ def main():
     avg_launch_ms = trace_ms / max(trace_launches, 1)
     rays_per_launch = trace_rays / max(trace_launches, 1)
     achieved = bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_extend (FP32 closest-hit traversal)",
+                "traffic": traffic, "kernel": "k_extend (FP32 closest-hit traversal)",
                 "algorithmic_bytes_per_ray": bytes_per_ray, "avg_launch_ms": avg_launch_ms,
-                "rays_per_launch": rays_per_launch}
+                "rays_per_launch": rays_per_launch,
+                "traffic_source": TRAFFIC_PROFILE if traffic else None,
+                "hbm_gbs_from_traffic": traffic / (avg_launch_ms * 1e-3) / 1e9 if traffic else None}
 
     line = {"metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": secs * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
