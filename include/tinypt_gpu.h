@@ -207,6 +207,17 @@ int tpt_render(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cf
 int tpt_render_device(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg,
                       double* d_acc_rgb, double* d_acc_w, double* d_acc_sq, tpt_stats* stats);
 
+/* A batch of tile tasks (render.rs:119-198, task.rs:9-40): every listed image pixel (index
+ * y * width + x) is rendered with all cfg->spp samples -- adaptive and either precision as cfg
+ * says -- and its sums are written, in list order, to sum_rgb[3i..3i+2], sum_w[i] and
+ * sum_sq[3i..3i+2] (host pointers; sum_sq may be null).  A pixel's samples do not depend on
+ * which other pixels share the call, so merging the sums in task order gives the full render
+ * (bit for bit with the plain FP64 form, and with the FP32 form when cfg->samples_per_pass is
+ * fixed, since its FP64 sums are then grouped alike).  The tiled, checkpointed render loop of
+ * render.rs is built on this; indices outside the image are rejected. */
+int tpt_render_pixels(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, int64_t n,
+                      const int32_t* pixels, double* sum_rgb, double* sum_w, double* sum_sq, tpt_stats* stats);
+
 /* Closest hit of n rays (World::hit, world.rs:81-121) in FP32.  rays[n*8]:
  * ox oy oz dx dy dz tmin tmax (d need not be unit length; it is used as given).  Outputs per
  * ray: hit_t (world t, +inf on miss), hit_kind (0 miss, 1 triangle, 2 sphere), hit_inst
@@ -233,6 +244,31 @@ int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_co
 /* Update the material table of an uploaded scene in place (mat_param[n_materials*4]); the
  * light CDF is rebuilt when emitter radiance changes.  Used by inverse rendering loops. */
 int tpt_scene_update_materials(tpt_ctx* ctx, tpt_scene* scene, const double* mat_param);
+
+/* ---- output and checkpoints --------------------------------------------------------------- */
+typedef enum tpt_tonemap {
+  TPT_TONEMAP_NONE = 0,  /* Tonemap::None (config.rs:50-56) */
+  TPT_TONEMAP_ACES = 1   /* Tonemap::Aces: Narkowicz fit (output.rs:137-148) */
+} tpt_tonemap;
+
+/* resolve_pixels (output.rs:96-102): rgb_out[i] = acc_rgb[i] / max(acc_w[i], 1), n pixels. */
+int tpt_resolve_pixels(int64_t n, const double* acc_rgb, const double* acc_w, double* rgb_out);
+
+/* OutputFormat::from_path + write (output.rs:45-94) on the accumulated buffers: ".exr" (any
+ * case) writes linear ACEScg float32 R, G, B with ACEScg chromaticities (an uncompressed
+ * OpenEXR file; the reference's encoder compresses), ".hdr" writes Radiance RGBE (hdr.rs:65-88),
+ * anything else an ASCII PPM with exposure (EV), the tonemap and the sRGB encoding. */
+int tpt_write_image(const char* path, int32_t width, int32_t height, const double* acc_rgb, const double* acc_w,
+                    double exposure, int32_t tonemap);
+
+/* save_checkpoint / load_checkpoint (checkpoint.rs:48-138): the HYPCKPT v1 file (magic,
+ * version, scene hash, width, height, next id, acc, acc_w; little endian), written through a
+ * temporary file and a rename.  load sets *found = 0 (and returns TPT_OK) when the file is
+ * absent or its magic, version, scene hash or size differ, as the reference returns None. */
+int tpt_checkpoint_save(const char* path, uint64_t scene_hash, int32_t width, int32_t height, uint64_t next_id,
+                        const double* acc_rgb, const double* acc_w);
+int tpt_checkpoint_load(const char* path, uint64_t scene_hash, int32_t width, int32_t height, uint64_t* next_id,
+                        double* acc_rgb, double* acc_w, int32_t* found);
 
 /* ---- scene files ------------------------------------------------------------------------- */
 typedef struct tpt_loaded_scene tpt_loaded_scene;

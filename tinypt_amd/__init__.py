@@ -27,7 +27,7 @@ from .scene import (Camera, DiffuseLight, Dielectric, EnvMap, Ggx, Instance, Lam
 __all__ = ["Camera", "DiffuseLight", "Dielectric", "EnvMap", "Ggx", "Instance", "Lambert", "Material", "Mesh", "Metal",
            "RenderConfig", "Scene", "Sphere", "Subsurface", "Transform", "World", "Context", "DeviceScene", "HostScene",
            "RenderOutput",
-           "render", "render_device", "render_adjoint", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
+           "render", "render_pixels", "render_device", "render_adjoint", "resolve_pixels", "lib_path", "load_library", "TptError", "FP32_WAVEFRONT", "FP64_PLAIN"]
 
 import atexit
 import weakref
@@ -238,10 +238,20 @@ class RenderOutput:
     stats: dict
 
 
-def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Context] = None) -> RenderOutput:
+def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Context] = None,
+           ckpt_file: Optional[str] = None) -> RenderOutput:
     """render() (render.rs:42-278) on the GPU.  ``scene`` is a Scene or an uploaded
-    DeviceScene.  Checkpointing is the caller's business (the buffers are returned)."""
+    DeviceScene.
+
+    With ``ckpt_file`` set, ``config.checkpoint_enabled`` and ``checkpoint_every_tasks`` > 0 the
+    render follows the reference's tile tasks: it resumes from a matching checkpoint, renders
+    the remaining tiles in task order (batches of whole checkpoint intervals, one GPU call per
+    batch), merges them in task-id order and saves the HYPCKPT file every
+    ``checkpoint_every_tasks`` tasks and at the end.  ``acc_sq`` is then None when the render
+    resumed (the file holds no squares).  Otherwise the whole image is one GPU call."""
     dscene = scene if isinstance(scene, DeviceScene) else DeviceScene(scene, ctx)
+    if ckpt_file is not None and config.checkpoint_enabled and config.checkpoint_every_tasks > 0:
+        return _render_checkpointed(dscene, config, ckpt_file, want_sq)
     lib = dscene._lib
     h, w = int(config.height), int(config.width)
     acc = np.zeros((h, w, 3))
@@ -252,6 +262,77 @@ def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Cont
     _check(lib.tpt_render(dscene.ctx.h, dscene.h, C.byref(cfg), _dp(acc), _dp(acc_w),
                           _dp(acc_sq) if want_sq else None, C.byref(st)))
     return RenderOutput(acc, acc_w, acc_sq, st.as_dict())
+
+
+def render_pixels(scene: "DeviceScene", config: RenderConfig, pixels: np.ndarray, want_sq: bool = False) -> tuple:
+    """``tpt_render_pixels``: every listed image pixel (y * width + x) rendered with all its
+    samples; returns (sum (n, 3), sum_w (n,), sum_sq (n, 3) or None, counters) in list order
+    (the TileResult sums of task.rs:28-40)."""
+    pix = np.ascontiguousarray(pixels, dtype=np.int32).reshape(-1)
+    n = pix.shape[0]
+    s = np.zeros((n, 3))
+    sw = np.zeros(n)
+    sq = np.zeros((n, 3)) if want_sq else None
+    st = _abi.Stats()
+    cfg = config.to_c()
+    _check(scene._lib.tpt_render_pixels(scene.ctx.h, scene.h, C.byref(cfg), n, pix.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        _dp(s), _dp(sw), _dp(sq) if want_sq else None, C.byref(st)))
+    return s, sw, sq, st.as_dict()
+
+
+_CKPT_BATCH_PATHS = 1 << 22  # samples per GPU call below which more checkpoint intervals join a batch
+
+
+def _render_checkpointed(dscene: "DeviceScene", config: RenderConfig, ckpt_file: str, want_sq: bool) -> RenderOutput:
+    """render.rs:48-278 with checkpoints: tiles in task-id order, merged in that order."""
+    import sys
+    from .output import load_checkpoint, save_checkpoint, task_pixels, tile_tasks
+    h, w = int(config.height), int(config.width)
+    acc = np.zeros((h, w, 3))
+    acc_w = np.zeros((h, w))
+    resume = 0
+    try:
+        got = load_checkpoint(ckpt_file, config.scene_hash, w, h)
+    except TptError:
+        got = None  # render.rs:57 takes only Ok(Some(..))
+    if got is not None:
+        resume, acc, acc_w = got
+        print(f"Resumed from checkpoint: {ckpt_file} (next task id: {resume})", file=sys.stderr)
+    acc_sq = np.zeros((h, w, 3)) if want_sq and resume == 0 else None
+    tasks = tile_tasks(w, h, int(config.tile), bool(config.morton_enabled))
+    tid = len(tasks)
+    every = int(config.checkpoint_every_tasks)
+    flat_acc, flat_w = acc.reshape(-1, 3), acc_w.reshape(-1)
+    flat_sq = acc_sq.reshape(-1, 3) if acc_sq is not None else None
+    stats = {}
+    next_id = min(resume, tid)
+    sizes = [(x1 - x0) * (y1 - y0) for x0, y0, x1, y1 in tasks]
+    while next_id < tid:
+        end, n_pix = next_id, 0
+        while end < tid:
+            stop = min(tid, (end // every + 1) * every)
+            n_pix += sum(sizes[end:stop])
+            end = stop
+            if n_pix * max(int(config.spp), 1) >= _CKPT_BATCH_PATHS:
+                break
+        pix = task_pixels(tasks[next_id:end], w)
+        s, sw, sq, st = render_pixels(dscene, config, pix, want_sq=flat_sq is not None)
+        for k, v in st.items():
+            stats[k] = stats.get(k, 0) + v
+        off = 0
+        for t in range(next_id, end):
+            sl = slice(off, off + sizes[t])
+            p = pix[sl]
+            flat_acc[p] = flat_acc[p] + s[sl]
+            flat_w[p] += sw[sl]
+            if flat_sq is not None:
+                flat_sq[p] += sq[sl]
+            off += sizes[t]
+            if (t + 1) % every == 0:
+                save_checkpoint(ckpt_file, config.scene_hash, t + 1, acc, acc_w)
+        next_id = end
+    save_checkpoint(ckpt_file, config.scene_hash, tid, acc, acc_w)  # render.rs:267-272
+    return RenderOutput(acc, acc_w, acc_sq, stats)
 
 
 def render_device(scene: "DeviceScene", config: RenderConfig, acc, acc_w, acc_sq=None) -> dict:

@@ -144,4 +144,13 @@ ENTRY_POINTS = {
     "tpt_loaded_scene_warnings": (C.c_char_p, [_vp]),
     "tpt_loaded_scene_destroy": (C.c_int, [_vp]),
     "tpt_obj_load": (C.c_int, [C.c_char_p, _lp, _dp]),
+    "tpt_render_pixels": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), C.c_int64, _ip, _dp, _dp, _dp,
+                                    C.POINTER(Stats)]),
+    "tpt_resolve_pixels": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
+    "tpt_write_image": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _dp, _dp, C.c_double, C.c_int32]),
+    "tpt_checkpoint_save": (C.c_int, [C.c_char_p, C.c_uint64, C.c_int32, C.c_int32, C.c_uint64, _dp, _dp]),
+    "tpt_checkpoint_load": (C.c_int, [C.c_char_p, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_uint64), _dp, _dp,
+                                      _ip]),
 }
+TONEMAP_NONE = 0
+TONEMAP_ACES = 1

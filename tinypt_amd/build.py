@@ -41,6 +41,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=o
 HOST_SOURCES = [
     ("scene_host.cpp", HOST_FLAGS),
     ("scene_xml.cpp", HOST_FLAGS),
+    ("output.cpp", HOST_FLAGS),
 ]
 
 

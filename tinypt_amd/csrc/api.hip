@@ -508,10 +508,11 @@ int wf_bounces(tpt_ctx* c, const WfParams& W, uint32_t P, EventTimer& tt, uint64
   return TPT_OK;
 }
 
-int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, double* acc, double* acc_w,
-                     double* acc_sq, tpt_stats* st) {
+// Renders the n_pix pixels of the domain: pixel q of the domain is image pixel pix_map[q]
+// (q itself when pix_map is null), and its sums go to acc[3q..], acc_w[q], acc_sq[3q..].
+int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, uint32_t npix,
+                     const int32_t* pix_map, double* acc, double* acc_w, double* acc_sq, tpt_stats* st) {
   hipStream_t s = c->stream;
-  const uint32_t npix = (uint32_t)cfg->width * (uint32_t)cfg->height;
   const uint32_t spp = (uint32_t)cfg->spp;
   const uint32_t target = 1u << 24;  // paths in flight (about 3.8 GB of path state)
   uint32_t k = cfg->samples_per_pass > 0 ? (uint32_t)cfg->samples_per_pass : std::max(1u, target / std::max(npix, 1u));
@@ -536,7 +537,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       uint32_t np = std::min(pix_chunk, npix - p0);
       for (uint32_t s0 = 0; s0 < spp; s0 += k) {
         uint32_t kk = std::min(k, spp - s0);
-        RaygenArgs A{np * kk, np, p0, s0, nullptr, nullptr};
+        RaygenArgs A{np * kk, np, p0, s0, nullptr, nullptr, pix_map};
         HIPC(wf_launch_raygen(W, A, s));
         rc = wf_bounces(c, W, np * kk, tt, &launches);
         if (rc) return rc;
@@ -568,7 +569,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       kk = std::max(kk, std::min<uint32_t>(fill, 64));
       kk = std::min(kk, spp - taken);
       kk = std::min<uint32_t>(kk, (uint32_t)std::max<size_t>(1, P / n_active));
-      RaygenArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>()};
+      RaygenArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>(), pix_map};
       HIPC(wf_launch_raygen(W, A, s));
       rc = wf_bounces(c, W, n_active * kk, tt, &launches);
       if (rc) return rc;
@@ -601,15 +602,14 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   return TPT_OK;
 }
 
-int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, double* acc, double* acc_w, double* acc_sq,
-                 tpt_stats* st) {
+int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, uint32_t npix, const int32_t* pix_map,
+                 double* acc, double* acc_w, double* acc_sq, tpt_stats* st) {
   if (sc->host.max_blas_depth > 63)
     return fail(TPT_ERR_UNSUPPORTED, "the plain FP64 form keeps the reference's 64-entry BVH stack; tree too deep");
   hipStream_t s = c->stream;
-  const size_t npix = (size_t)cfg->width * cfg->height;
-  HIPC(c->rngpix.ensure(npix * 8));
-  HIPC(c->nsamp.ensure(npix * 4));
-  HIPC(c->welford.ensure(npix * 16));
+  HIPC(c->rngpix.ensure((size_t)npix * 8));
+  HIPC(c->nsamp.ensure((size_t)npix * 4));
+  HIPC(c->welford.ensure((size_t)npix * 16));
   HIPC(c->done.ensure(npix));
   HIPC(c->stats.ensure(64));
   HIPC(hipMemsetAsync(c->done.p, 0, npix, s));
@@ -622,6 +622,8 @@ int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   P.cam = sc->host.cam;
   P.width = cfg->width;
   P.height = cfg->height;
+  P.n_pix = (int32_t)npix;
+  P.pix_map = pix_map;
   P.spp = cfg->spp;
   P.max_bounces = cfg->max_bounces;
   P.rr_start = cfg->rr_start;
@@ -668,16 +670,31 @@ int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   return TPT_OK;
 }
 
-int render_device(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, double* acc, double* acc_w, double* acc_sq,
-                  tpt_stats* st) {
-  const size_t npix = (size_t)cfg->width * cfg->height;
+// Zeroes the npix-entry sums, then renders the domain (see render_wavefront).
+int render_device(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, uint32_t npix, const int32_t* pix_map,
+                  double* acc, double* acc_w, double* acc_sq, tpt_stats* st) {
   hipStream_t s = c->stream;
-  HIPC(hipMemsetAsync(acc, 0, npix * 24, s));
-  HIPC(hipMemsetAsync(acc_w, 0, npix * 8, s));
-  if (acc_sq) HIPC(hipMemsetAsync(acc_sq, 0, npix * 24, s));
+  HIPC(hipMemsetAsync(acc, 0, (size_t)npix * 24, s));
+  HIPC(hipMemsetAsync(acc_w, 0, (size_t)npix * 8, s));
+  if (acc_sq) HIPC(hipMemsetAsync(acc_sq, 0, (size_t)npix * 24, s));
   if (st) std::memset(st, 0, sizeof(*st));
-  if (cfg->precision == TPT_FP64_PLAIN) return render_plain(c, sc, cfg, acc, acc_w, acc_sq, st);
-  return render_wavefront(c, sc, cfg, acc, acc_w, acc_sq, st);
+  if (cfg->precision == TPT_FP64_PLAIN) return render_plain(c, sc, cfg, npix, pix_map, acc, acc_w, acc_sq, st);
+  return render_wavefront(c, sc, cfg, npix, pix_map, acc, acc_w, acc_sq, st);
+}
+
+// Host-side sums for a domain of npix pixels: device sums, render, copy back.
+int render_to_host(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, uint32_t npix, const int32_t* pix_map,
+                   double* acc_rgb, double* acc_w, double* acc_sq, tpt_stats* st) {
+  DevBuf a, w, q;
+  HIPC(a.ensure((size_t)npix * 24));
+  HIPC(w.ensure((size_t)npix * 8));
+  if (acc_sq) HIPC(q.ensure((size_t)npix * 24));
+  int rc = render_device(c, sc, cfg, npix, pix_map, a.as<double>(), w.as<double>(), acc_sq ? q.as<double>() : nullptr, st);
+  if (rc) return rc;
+  HIPC(hipMemcpy(acc_rgb, a.p, (size_t)npix * 24, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(acc_w, w.p, (size_t)npix * 8, hipMemcpyDeviceToHost));
+  if (acc_sq) HIPC(hipMemcpy(acc_sq, q.p, (size_t)npix * 24, hipMemcpyDeviceToHost));
+  return TPT_OK;
 }
 
 }  // namespace
@@ -691,7 +708,28 @@ int tpt_render_device(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_con
   int rc = check_cfg(cfg);
   if (rc) return rc;
   HIPC(hipSetDevice(ctx->device));
-  return render_device(ctx, scene, cfg, d_acc_rgb, d_acc_w, d_acc_sq, stats);
+  return render_device(ctx, scene, cfg, (uint32_t)cfg->width * (uint32_t)cfg->height, nullptr, d_acc_rgb, d_acc_w,
+                       d_acc_sq, stats);
+}
+
+int tpt_render_pixels(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, int64_t n,
+                      const int32_t* pixels, double* sum_rgb, double* sum_w, double* sum_sq, tpt_stats* stats) {
+  if (!ctx || !scene || n < 0 || (n > 0 && (!pixels || !sum_rgb || !sum_w)))
+    return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (scene->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  const int64_t npix = (int64_t)cfg->width * cfg->height;
+  if (n > npix) return fail(TPT_ERR_INVALID_ARGUMENT, "more pixels listed than the image holds");
+  for (int64_t i = 0; i < n; i++)
+    if (pixels[i] < 0 || pixels[i] >= npix) return fail(TPT_ERR_INVALID_ARGUMENT, "pixel index outside the image");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return TPT_OK;
+  HIPC(hipSetDevice(ctx->device));
+  DevBuf map;
+  HIPC(map.ensure((size_t)n * 4));
+  HIPC(hipMemcpy(map.p, pixels, (size_t)n * 4, hipMemcpyHostToDevice));
+  return render_to_host(ctx, scene, cfg, (uint32_t)n, map.as<int32_t>(), sum_rgb, sum_w, sum_sq, stats);
 }
 
 int tpt_render(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, double* acc_rgb, double* acc_w,
@@ -701,17 +739,8 @@ int tpt_render(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cf
   int rc = check_cfg(cfg);
   if (rc) return rc;
   HIPC(hipSetDevice(ctx->device));
-  const size_t npix = (size_t)cfg->width * cfg->height;
-  DevBuf a, w, q;
-  HIPC(a.ensure(npix * 24));
-  HIPC(w.ensure(npix * 8));
-  if (acc_sq) HIPC(q.ensure(npix * 24));
-  rc = render_device(ctx, scene, cfg, a.as<double>(), w.as<double>(), acc_sq ? q.as<double>() : nullptr, stats);
-  if (rc) return rc;
-  HIPC(hipMemcpy(acc_rgb, a.p, npix * 24, hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(acc_w, w.p, npix * 8, hipMemcpyDeviceToHost));
-  if (acc_sq) HIPC(hipMemcpy(acc_sq, q.p, npix * 24, hipMemcpyDeviceToHost));
-  return TPT_OK;
+  return render_to_host(ctx, scene, cfg, (uint32_t)cfg->width * (uint32_t)cfg->height, nullptr, acc_rgb, acc_w, acc_sq,
+                        stats);
 }
 
 int tpt_trace_closest(tpt_ctx* ctx, const tpt_scene* sc, int64_t n, const float* rays, float* hit_t, int32_t* hit_kind,

@@ -408,12 +408,12 @@ __device__ d3 radiance64(const Plain64Params& P, d3 ro, d3 rd, double time, RngF
 
 // ---------------------------------------------------------------- render.rs:137-191
 __global__ void __launch_bounds__(TPT_PLAIN_BLOCK) k_plain64(Plain64Params P) {
-  const int npix = P.width * P.height;
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;  // domain pixel: indexes the sums and state
   Count64 cnt{0, 0, 0};
   uint32_t paths = 0;
-  if (pix < npix && !P.done[pix]) {
-    const int x = pix % P.width, y = pix / P.width;
+  if (pix < P.n_pix && !P.done[pix]) {
+    const int img = P.pix_map ? P.pix_map[pix] : pix;
+    const int x = img % P.width, y = img / P.width;
     RngF64 rng(1);
     rng.p.state = P.s_begin == 0 ? Pcg32(seed_for((uint32_t)x, (uint32_t)y, 0) ^ P.seed).state : P.rng[pix];
     d3 c = P.s_begin == 0 ? mk(0.0, 0.0, 0.0) : ld3(P.acc + 3 * pix);
@@ -485,8 +485,8 @@ __global__ void __launch_bounds__(TPT_PLAIN_BLOCK) k_plain64(Plain64Params P) {
 }
 
 hipError_t launch_plain64(const Plain64Params& P, hipStream_t s) {
-  int npix = P.width * P.height;
-  hipLaunchKernelGGL(k_plain64, dim3((npix + TPT_PLAIN_BLOCK - 1) / TPT_PLAIN_BLOCK), dim3(TPT_PLAIN_BLOCK), 0, s, P);
+  if (P.n_pix <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_plain64, dim3((P.n_pix + TPT_PLAIN_BLOCK - 1) / TPT_PLAIN_BLOCK), dim3(TPT_PLAIN_BLOCK), 0, s, P);
   return hipGetLastError();
 }
 

@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
       pix = A.pixel0 + i % A.n_pixels;
       s = A.s0 + i / A.n_pixels;
     }
+    if (A.pix_map) pix = (uint32_t)A.pix_map[pix];
     uint32_t x = pix % (uint32_t)W.width, y = pix / (uint32_t)W.width;
     RngF32 rng(seed_for(x, y, s) ^ W.seed);
     f3 origin, dir;

@@ -17,6 +17,8 @@ struct Plain64Params {
   MatsDev mats;
   CameraF64 cam;
   int32_t width, height, spp, max_bounces, rr_start, adaptive, min_spp;
+  int32_t n_pix;            // pixels of the domain; domain pixel q is image pixel pix_map[q] (q if null)
+  const int32_t* pix_map;
   int32_t s_begin, s_end;  // samples of this launch
   double threshold;
   double inv_w, inv_h;

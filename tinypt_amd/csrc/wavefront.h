@@ -66,6 +66,7 @@ struct RaygenArgs {
   uint32_t s0;        // first sample index (non-adaptive)
   const int32_t* active;  // adaptive: active pixel list (else null)
   const uint32_t* nsamp;  // adaptive: samples taken per pixel
+  const int32_t* pix_map;  // image pixel of each domain pixel (null: the domain is the image)
 };
 
 struct AccumArgs {

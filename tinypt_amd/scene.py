@@ -366,6 +366,13 @@ class RenderConfig:
     seed: int = 0
     precision: int = _abi.FP32_WAVEFRONT
     samples_per_pass: int = 0
+    # host-side settings (config.rs:17-45, 79-86): checkpoints and image output
+    checkpoint_enabled: bool = False
+    checkpoint_every_tasks: int = 128
+    scene_hash: int = 0x4859503000000001
+    output_path: str = "out.ppm"
+    tonemap: str = "aces"
+    exposure: float = 0.0
 
     def to_c(self) -> _abi.RenderConfigC:
         c = _abi.RenderConfigC()
