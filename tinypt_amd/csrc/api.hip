@@ -399,27 +399,29 @@ int needed_gstack(const HostScene& H) {
   return need > TPT_WF_STACK ? need - TPT_WF_STACK : 0;
 }
 
+// Queue arrays hold TPT_WF_SEGS segments of wf_seg_cap(P) entries; hit and L are dense.
 int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
   if (P > c->ps_cap) {
+    const size_t Q = (size_t)TPT_WF_SEGS * wf_seg_cap((uint32_t)P);
     for (int b = 0; b < 2; b++) {
-      HIPC(c->ps_o[b].ensure(P * 16));
-      HIPC(c->ps_d[b].ensure(P * 16));
-      HIPC(c->ps_beta[b].ensure(P * 16));
-      HIPC(c->ps_rng[b].ensure(P * 8));
+      HIPC(c->ps_o[b].ensure(Q * 16));
+      HIPC(c->ps_d[b].ensure(Q * 16));
+      HIPC(c->ps_beta[b].ensure(Q * 16));
+      HIPC(c->ps_rng[b].ensure(Q * 8));
     }
     HIPC(c->ps_hit.ensure(P * 16));
     HIPC(c->ps_L.ensure(P * 16));
     for (int q = 0; q < 2; q++) {
-      HIPC(c->sh_o[q].ensure(P * 16));
-      HIPC(c->sh_d[q].ensure(P * 16));
-      HIPC(c->sh_c[q].ensure(P * 16));
+      HIPC(c->sh_o[q].ensure(Q * 16));
+      HIPC(c->sh_d[q].ensure(Q * 16));
+      HIPC(c->sh_c[q].ensure(Q * 16));
     }
-    HIPC(c->sh_l.ensure(P * 16));
-    HIPC(c->sh_lv.ensure(P * 8));
+    HIPC(c->sh_l.ensure(Q * 16));
+    HIPC(c->sh_lv.ensure(Q * 8));
     c->ps_cap = P;
   }
   HIPC(c->counters.ensure(TPT_NUM_CNT * 4));
-  HIPC(c->stats.ensure(8 * 8));
+  HIPC(c->stats.ensure(TPT_STAT_WORDS * 8));
   if (gcap > 0) HIPC(c->gstack.ensure((size_t)wf_grid_threads((uint32_t)P) * (size_t)gcap * 4));
   return TPT_OK;
 }
@@ -450,6 +452,7 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   ps.sh_lv = c->sh_lv.as<float2>();
   ps.counters = c->counters.as<uint32_t>();
   ps.stats = c->stats.as<uint64_t>();
+  ps.seg_cap = wf_seg_cap((uint32_t)c->ps_cap);
   W.gstack = gcap > 0 ? c->gstack.as<int32_t>() : nullptr;
   W.gstack_cap = gcap;
   W.width = cfg->width;
@@ -528,7 +531,8 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   int rc = ensure_paths(c, P, gcap);
   if (rc) return rc;
   WfParams W = make_wf(c, sc, cfg, gcap);
-  HIPC(hipMemsetAsync(W.ps.stats, 0, 8 * 8, s));
+  HIPC(hipMemsetAsync(W.ps.stats, 0, TPT_STAT_WORDS * 8, s));
+  HIPC(hipMemsetAsync(W.ps.counters, 0, TPT_NUM_CNT * 4, s));
   EventTimer total(s, st != nullptr), tt(s, st != nullptr);
   uint64_t launches = 0, traced = 0;
   HIPC(total.mark());
@@ -587,9 +591,15 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   }
   HIPC(total.mark());
   HIPC(hipStreamSynchronize(s));
+  uint32_t overflow = 0;
+  HIPC(hipMemcpy(&overflow, W.ps.counters + TPT_CNT_OVERFLOW, 4, hipMemcpyDeviceToHost));
+  if (overflow) return fail(TPT_ERR_INTERNAL, "a path queue segment overflowed");
   if (st) {
-    uint64_t h[8] = {0};
-    HIPC(hipMemcpy(h, W.ps.stats, 8 * 8, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> seg(TPT_STAT_WORDS);
+    HIPC(hipMemcpy(seg.data(), W.ps.stats, TPT_STAT_WORDS * 8, hipMemcpyDeviceToHost));
+    uint64_t h[3] = {0, 0, 0};
+    for (int g = 0; g < TPT_WF_SEGS; g++)
+      for (int k = 0; k < 3; k++) h[k] += seg[(size_t)g * TPT_STAT_STRIDE + k];
     st->paths = h[2];
     st->rays_extension = h[0];
     st->rays_shadow = h[1];

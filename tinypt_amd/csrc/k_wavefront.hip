@@ -7,7 +7,7 @@
 // sampling; survivors compacted into the next queue) -> shadow (any hit for the <= 2 NEE rays
 // of each path, added to that path's radiance by the thread that owns the path).
 // After the last bounce, accumulate adds each pixel's k path radiances into FP64 sums.
-// Queue compaction: one __ballot per wave, one atomicAdd per wave, lane prefix by popcount.
+// Queue compaction: a ballot per wave and one atomic per block on the block's queue segment.
 // Random numbers: one PCG32 stream per (pixel, sample): seed_for(x, y, s) ^ seed.
 #include "tpt_bsdf.h"
 #include "tpt_kernels.h"
@@ -40,6 +40,19 @@ __device__ __forceinline__ uint32_t block_slot(bool pred, uint32_t* __restrict__
   return pos;
 }
 
+// Slot of this lane's entry in queue q (segment blockIdx % TPT_WF_SEGS) when `pred` holds, or
+// UINT32_MAX (and the overflow flag raised) if the segment were full, which the segment size
+// rules out.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t seg_slot(bool pred, const PathState& ps, int q, uint32_t* sh) {
+  const uint32_t g = blockIdx.x & (TPT_WF_SEGS - 1);
+  const uint32_t pos = block_slot(pred, ps.counters + TPT_CNT_IDX(q, g), sh);
+  if (pred && pos >= ps.seg_cap) {
+    ps.counters[TPT_CNT_OVERFLOW] = 1u;
+    return UINT32_MAX;
+  }
+  return g * ps.seg_cap + pos;
+}
+
 // ---------------------------------------------------------------- kernels
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -60,35 +73,44 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
     RngF32 rng(seed_for(x, y, s) ^ W.seed);
     f3 origin, dir;
     camera_ray32(W.cam, x, y, W.inv_w, W.inv_h, rng, &origin, &dir);
-    W.ps.o[0][i] = make_float4(origin.x, origin.y, origin.z, 0.f);
-    W.ps.d[0][i] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
-    W.ps.beta[0][i] = make_float4(1.f, 1.f, 1.f, __int_as_float((int32_t)i));
-    W.ps.rng[0][i] = rng.p.state;
+    // chunk c of TPT_WF_BLOCK paths goes to segment c % SEGS, as the shading kernel's blocks
+    // would place it
+    const uint32_t c = i / TPT_WF_BLOCK;
+    const uint32_t slot = (c % TPT_WF_SEGS) * W.ps.seg_cap + (c / TPT_WF_SEGS) * TPT_WF_BLOCK + i % TPT_WF_BLOCK;
+    W.ps.o[0][slot] = make_float4(origin.x, origin.y, origin.z, 0.f);
+    W.ps.d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
+    W.ps.beta[0][slot] = make_float4(1.f, 1.f, 1.f, __int_as_float((int32_t)i));
+    W.ps.rng[0][slot] = rng.p.state;
     W.ps.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  if (gtid() == 0) {
-    W.ps.counters[TPT_CNT_Q0] = P;
-    W.ps.counters[TPT_CNT_Q1] = 0;
-    W.ps.counters[TPT_CNT_SH] = 0;
-    W.ps.counters[TPT_CNT_SHL] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < TPT_WF_SEGS) {
+    const uint32_t g = threadIdx.x, full = P / TPT_WF_BLOCK, rem = P % TPT_WF_BLOCK;
+    const uint32_t n_full = full > g ? (full - g + TPT_WF_SEGS - 1) / TPT_WF_SEGS : 0u;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_Q0, g)] = n_full * TPT_WF_BLOCK + (rem && full % TPT_WF_SEGS == g ? rem : 0u);
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_Q1, g)] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, g)] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, g)] = 0;
   }
 }
 
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_extend(WfParams W, int cur) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
-  const uint32_t n = W.ps.counters[cur ? TPT_CNT_Q1 : TPT_CNT_Q0];
+  QueueMap qm;
+  qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
+  const uint32_t n = qm.n;
   const float4* __restrict__ po = W.ps.o[cur];
   const float4* __restrict__ pd = W.ps.d[cur];
   LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
                TPT_WF_STACK};
-  if (gtid() == 0) {
-    W.ps.counters[cur ? TPT_CNT_Q0 : TPT_CNT_Q1] = 0;  // next queue
-    W.ps.counters[TPT_CNT_SH] = 0;
-    W.ps.counters[TPT_CNT_SHL] = 0;
-    atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
+  if (gtid() < TPT_WF_SEGS) {  // clear the queues the shading kernel fills next
+    W.ps.counters[TPT_CNT_IDX(cur ? TPT_CNT_Q0 : TPT_CNT_Q1, gtid())] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, gtid())] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, gtid())] = 0;
   }
+  if (gtid() == 0) atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
   for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
-    float4 o4 = po[i], d4 = pd[i];
+    const uint32_t slot = qm.slot(i);
+    float4 o4 = po[slot], d4 = pd[slot];
     HitF32 h;
     trace_f32<false>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, 1e30f, &h, st);
     W.ps.hit[i] = make_int4(__float_as_int(h.t), h.inst, h.prim, 0);
@@ -102,11 +124,10 @@ __device__ __forceinline__ void add_radiance(float4* L, int32_t id, f3 c) {
 
 __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shade(WfParams W, int cur) {
   __shared__ uint32_t s_slot[8];
-  const uint32_t n = W.ps.counters[cur ? TPT_CNT_Q1 : TPT_CNT_Q0];
+  QueueMap qm;
+  qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
+  const uint32_t n = qm.n;
   const int nxt = cur ^ 1;
-  uint32_t* cout = &W.ps.counters[cur ? TPT_CNT_Q0 : TPT_CNT_Q1];
-  uint32_t* csh_env = &W.ps.counters[TPT_CNT_SH];
-  uint32_t* csh_light = &W.ps.counters[TPT_CNT_SHL];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
@@ -119,7 +140,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     RngF32 rng(1);
     int4 h4 = make_int4(0, -1, 0, 0);
     if (i < n) {
-      float4 o4 = W.ps.o[cur][i], d4 = W.ps.d[cur][i], b4 = W.ps.beta[cur][i];
+      const uint32_t slot = qm.slot(i);
+      float4 o4 = W.ps.o[cur][slot], d4 = W.ps.d[cur][slot], b4 = W.ps.beta[cur][slot];
       h4 = W.ps.hit[i];
       o = f3of(o4);
       d = f3of(d4);
@@ -127,7 +149,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
       id = __float_as_int(b4.w);
       last_pdf = o4.w;
       flags = __float_as_uint(d4.w);
-      rng.p.state = W.ps.rng[cur][i];
+      rng.p.state = W.ps.rng[cur][slot];
     }
     const int bounce = (int)(flags & 0xffu);
     const bool last_nd = (flags & 0x100u) != 0;
@@ -214,14 +236,14 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
         }
       }
     }
-    const uint32_t je = block_slot(has_env, csh_env, s_slot);
-    if (has_env) {
+    const uint32_t je = seg_slot(has_env, W.ps, TPT_CNT_SH, s_slot);
+    if (has_env && je != UINT32_MAX) {
       W.ps.sh_o[0][je] = sh_o0;
       W.ps.sh_d[0][je] = sh_d0;
       W.ps.sh_c[0][je] = sh_c0;
     }
-    const uint32_t jl = block_slot(has_light, csh_light, s_slot);
-    if (has_light) {
+    const uint32_t jl = seg_slot(has_light, W.ps, TPT_CNT_SHL, s_slot);
+    if (has_light && jl != UINT32_MAX) {
       W.ps.sh_o[1][jl] = sh_o1;
       W.ps.sh_d[1][jl] = sh_d1;
       W.ps.sh_c[1][jl] = sh_c1;
@@ -241,8 +263,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
         alive = true;
       }
     }
-    const uint32_t k = block_slot(alive, cout, s_slot);
-    if (alive) {
+    const uint32_t k = seg_slot(alive, W.ps, cur ? TPT_CNT_Q0 : TPT_CNT_Q1, s_slot);
+    if (alive && k != UINT32_MAX) {
       uint32_t nflags = (uint32_t)(bounce + 1) | (bsdf_nd ? 0x100u : 0u);
       W.ps.o[nxt][k] = make_float4(o.x, o.y, o.z, last_pdf);
       W.ps.d[nxt][k] = make_float4(d.x, d.y, d.z, __uint_as_float(nflags));
@@ -256,12 +278,14 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
 // in FP64 (tpt_selfocc64.h) here, in a kernel of its own so the FP64 registers do not weigh on
 // k_shade.  A self-occluded ray is dropped before k_shadow traces it.
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
-  const uint32_t n = W.ps.counters[TPT_CNT_SHL];
-  for (uint32_t j = gtid(); j < n; j += gridDim.x * blockDim.x) {
-    float4 l = W.ps.sh_l[j];
-    float2 lv = W.ps.sh_lv[j];
+  QueueMap qm;
+  qm.load(W.ps.counters, TPT_CNT_SHL, W.ps.seg_cap);
+  for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
+    const uint32_t slot = qm.slot(j);
+    float4 l = W.ps.sh_l[slot];
+    float2 lv = W.ps.sh_lv[slot];
     if (light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x))
-      W.ps.sh_c[1][j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      W.ps.sh_c[1][slot] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -270,17 +294,19 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
 // read-modify-write of L[id] has a single writer per launch.
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
-  const uint32_t n = W.ps.counters[queue ? TPT_CNT_SHL : TPT_CNT_SH];
+  QueueMap qm;
+  qm.load(W.ps.counters, queue ? TPT_CNT_SHL : TPT_CNT_SH, W.ps.seg_cap);
   const float4* __restrict__ qo = W.ps.sh_o[queue];
   const float4* __restrict__ qd = W.ps.sh_d[queue];
   const float4* __restrict__ qc = W.ps.sh_c[queue];
   LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
                TPT_WF_STACK};
   uint32_t traced = 0;
-  for (uint32_t j = gtid(); j < n; j += gridDim.x * blockDim.x) {
-    const float4 c = qc[j];
+  for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
+    const uint32_t slot = qm.slot(j);
+    const float4 c = qc[slot];
     if (c.w == 0.f) continue;  // dropped (self-occluded light sample)
-    const float4 o4 = qo[j], d4 = qd[j];
+    const float4 o4 = qo[slot], d4 = qd[slot];
     HitF32 h;
     traced++;
     if (!trace_f32<true>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, o4.w, &h, st))
@@ -289,7 +315,9 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) 
   // one atomic per wave for the shadow-ray count
   uint32_t tot = traced;
   for (int off = 32; off > 0; off >>= 1) tot += (uint32_t)__shfl_down((int)tot, off);
-  if (__lane_id() == 0 && tot) atomicAdd((unsigned long long*)&W.ps.stats[1], (unsigned long long)tot);
+  if (__lane_id() == 0 && tot)
+    atomicAdd((unsigned long long*)&W.ps.stats[(blockIdx.x & (TPT_WF_SEGS - 1)) * TPT_STAT_STRIDE + 1],
+              (unsigned long long)tot);
 }
 
 // Non-adaptive: pixel p accumulates the k paths of this pass (slots p + j * pixels).
@@ -359,16 +387,20 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W
     if (keep) A.active_out[q] = pix;
   }
   for (int off = 32; off > 0; off >>= 1) kept += (uint32_t)__shfl_down((int)kept, off);
-  if (__lane_id() == 0 && kept) atomicAdd((unsigned long long*)&W.ps.stats[2], (unsigned long long)kept);
+  if (__lane_id() == 0 && kept)
+    atomicAdd((unsigned long long*)&W.ps.stats[(blockIdx.x & (TPT_WF_SEGS - 1)) * TPT_STAT_STRIDE + 2],
+              (unsigned long long)kept);
 }
 
 // ---------------------------------------------------------------- host-side launchers
 // Grid of at most TPT_WF_MAX_BLOCKS blocks: the kernels loop over their queues, so the grid only
 // needs to fill the chip (256 CUs x 8 blocks); larger grids cost dispatch time per bounce,
 // most of all for launches whose queues have shrunk to a few paths.
+// The grid is a multiple of TPT_WF_SEGS (queue segments follow blockIdx % TPT_WF_SEGS).
 static inline uint32_t grid_for(uint32_t n) {
   uint32_t b = (n + TPT_WF_BLOCK - 1) / TPT_WF_BLOCK;
-  return b < 1 ? 1 : (b > TPT_WF_MAX_BLOCKS ? TPT_WF_MAX_BLOCKS : b);
+  b = (b + TPT_WF_SEGS - 1) / TPT_WF_SEGS * TPT_WF_SEGS;
+  return b < TPT_WF_SEGS ? TPT_WF_SEGS : (b > TPT_WF_MAX_BLOCKS ? TPT_WF_MAX_BLOCKS : b);
 }
 
 hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s) {

@@ -15,17 +15,39 @@ static_assert(TPT_WF_BLOCK % 64 == 0 && TPT_WF_BLOCK / 64 <= 7, "block_slot keep
 #define TPT_WF_STACK 24      // LDS stack entries per lane (24 KB per 256-lane block)
 #define TPT_WF_STACK_ADJ 16  // the adjoint kernel keeps 16 in LDS (plus 4 KB of gradient slots)
 
+// Queues are split into TPT_WF_SEGS segments.  A block reserves its entries in segment
+// blockIdx % TPT_WF_SEGS -- with grids a multiple of TPT_WF_SEGS, the blocks of one XCD -- with
+// one atomic on that segment's counter.  Reservations that all hit one counter (or one cache
+// line) serialise at the memory side across the 8 XCDs and throttled the shading kernel; each
+// counter therefore sits on a 256 B line of its own.  Consumers see a queue as dense positions
+// 0..n-1 running through segment 0, then 1, ... (QueueMap).
+#define TPT_WF_SEGS 8
+#define TPT_CNT_STRIDE 64  // uint32 words between counters (256 B)
 #define TPT_CNT_Q0 0
 #define TPT_CNT_Q1 1
-#define TPT_CNT_SH 2  // environment shadow queue
-#define TPT_CNT_ACTIVE 3
-#define TPT_CNT_SHL 4  // area-light shadow queue
-#define TPT_NUM_CNT 8
+#define TPT_CNT_SH 2   // environment shadow queue
+#define TPT_CNT_SHL 3  // area-light shadow queue
+#define TPT_CNT_IDX(q, g) (((q) * TPT_WF_SEGS + (g)) * TPT_CNT_STRIDE)
+#define TPT_CNT_ACTIVE (4 * TPT_WF_SEGS * TPT_CNT_STRIDE)    // adaptive active-pixel list (one dense queue)
+#define TPT_CNT_OVERFLOW (TPT_CNT_ACTIVE + TPT_CNT_STRIDE)  // set if a segment would overflow (never by construction)
+#define TPT_NUM_CNT (TPT_CNT_OVERFLOW + TPT_CNT_STRIDE)
+#define TPT_STAT_STRIDE 32  // uint64 words of statistics per segment (256 B)
+#define TPT_STAT_WORDS (TPT_WF_SEGS * TPT_STAT_STRIDE)
+static_assert(TPT_WF_MAX_BLOCKS % TPT_WF_SEGS == 0, "grids are multiples of TPT_WF_SEGS");
 
-// Path state, structure of arrays.  The live paths of a bounce sit densely at queue positions
-// 0..n-1 of one of two buffer sets; shading writes the survivors densely into the other set, so
-// every kernel reads its inputs coalesced.  A path carries its id (the slot of its (pixel,
-// sample) in the pass) and gathers radiance in L[id].
+// Entries per queue segment for queues of at most P entries.  The shading kernel's block b
+// handles input chunks c (TPT_WF_BLOCK positions each) with c = b (mod grid), so with a grid that
+// is a multiple of TPT_WF_SEGS a segment receives the outputs of at most ceil(chunks / SEGS)
+// chunks; raygen lays the first queue out the same way.
+inline uint32_t wf_seg_cap(uint32_t P) {
+  const uint32_t chunks = (P + TPT_WF_BLOCK - 1) / TPT_WF_BLOCK;
+  return (chunks + TPT_WF_SEGS - 1) / TPT_WF_SEGS * TPT_WF_BLOCK;
+}
+
+// Path state, structure of arrays, in two buffer sets: the live paths of a bounce sit in the
+// segmented queue of one set; shading writes the survivors into the other set, so every kernel
+// reads its inputs coalesced.  A path carries its id (the slot of its (pixel, sample) in the
+// pass) and gathers radiance in L[id].  hit is indexed by dense queue position.
 struct PathState {
   float4* o[2];     // origin xyz, pdf of the last BSDF sample
   float4* d[2];     // direction xyz, flags (bits 0-7 bounce, bit 8 last bounce non-delta)
@@ -40,7 +62,37 @@ struct PathState {
   float4* sh_l;     // light queue: shading point xyz, light-sample u
   float2* sh_lv;    // light queue: light-sample v, light index (int bits)
   uint32_t* counters;
-  uint64_t* stats;  // [0] extension rays, [1] shadow rays, [2] paths
+  uint64_t* stats;  // per segment g: [g * TPT_STAT_STRIDE + k], k = 0 extension rays, 1 shadow rays, 2 paths
+  uint32_t seg_cap; // entries per queue segment (wf_seg_cap of the allocated path count)
+  uint32_t pad;
+};
+
+// Dense queue position -> slot of a segmented queue.
+struct QueueMap {
+  uint32_t pre[TPT_WF_SEGS];  // entries before segment g
+  uint32_t n;                 // entries in the queue
+  uint32_t cap;
+  __device__ __forceinline__ void load(const uint32_t* counters, int q, uint32_t seg_cap) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int g = 0; g < TPT_WF_SEGS; g++) {
+      pre[g] = s;
+      s += counters[TPT_CNT_IDX(q, g)];
+    }
+    n = s;
+    cap = seg_cap;
+  }
+  __device__ __forceinline__ uint32_t slot(uint32_t i) const {
+    uint32_t base = 0, off = i;
+#pragma unroll
+    for (int g = 1; g < TPT_WF_SEGS; g++) {
+      if (i >= pre[g]) {
+        base = (uint32_t)g * cap;
+        off = i - pre[g];
+      }
+    }
+    return base + off;
+  }
 };
 
 struct WfParams {
