@@ -9,8 +9,21 @@
 
 using namespace tpt;
 
+// The kernels' stack (a few "LDS" entries, the rest in the overflow area) with the deepest
+// entry count recorded, so tests can hold it against HostScene::stack4.
+struct CheckedStack {
+  LaneStack in;
+  int* max_sp;
+  void push(int& sp, int32_t v) const {
+    in.push(sp, v);
+    if (sp > *max_sp) *max_sp = sp;
+  }
+  int32_t pop(int& sp) const { return in.pop(sp); }
+};
+
 extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const float* rays, float* t, int32_t* kind,
-                                   int32_t* inst, int32_t* prim, int32_t* flat) {
+                                   int32_t* inst, int32_t* prim, int32_t* flat, int32_t* stack_bound,
+                                   int32_t* stack_max) {
   HostScene H;
   if (!H.build(d)) return -1;
   SceneF32 S{};
@@ -27,9 +40,12 @@ extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const flo
   S.n_spheres = (int32_t)H.sph_mat.size();
   S.flat_inst = H.flat ? H.flat_inst : -1;
   *flat = H.flat ? 1 : 0;
-  const int depth = H.max_tlas_depth + std::max(H.max_blas_depth, H.flat_depth) + 2;
-  std::vector<int32_t> stack((size_t)depth + 8);
-  LaneStack st{stack.data(), stack.data() + 4, 1, 4};  // 4 "LDS" entries, the rest overflow
+  // room far beyond the bound, so that a bound that is too small shows up in stack_max rather
+  // than as a write past the buffer
+  std::vector<int32_t> stack((size_t)H.stack4 + 4096);
+  int max_sp = 0;
+  CheckedStack st{LaneStack{stack.data(), stack.data() + 4, 1, 4}, &max_sp};  // 4 "LDS" entries, the rest overflow
+  *stack_bound = H.stack4;
   for (int64_t i = 0; i < n; i++) {
     const float* r = rays + 8 * i;
     HitF32 h;
@@ -42,6 +58,8 @@ extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const flo
       t[i] = h.t; kind[i] = TPT_HIT_TRI; inst[i] = (int32_t)((uint32_t)h.inst & ~TPT_HIT_FLAT);
       prim[i] = H.tri_orig[(size_t)h.prim];
     }
+    if (max_sp > H.stack4 + 4000) return -2;
   }
+  *stack_max = max_sp;
   return 0;
 }

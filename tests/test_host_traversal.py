@@ -33,11 +33,15 @@ def host_trace(sc, rays):
     n = len(r)
     t = np.zeros(n, np.float32)
     kind, inst, prim = (np.zeros(n, np.int32) for _ in range(3))
-    flat = C.c_int32()
+    flat, bound, deepest = C.c_int32(), C.c_int32(), C.c_int32()
     ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
     rc = _lib().tpt_test_host_trace(C.byref(desc), C.c_int64(n), r.ctypes.data_as(C.POINTER(C.c_float)),
-                                    t.ctypes.data_as(C.POINTER(C.c_float)), ip(kind), ip(inst), ip(prim), C.byref(flat))
+                                    t.ctypes.data_as(C.POINTER(C.c_float)), ip(kind), ip(inst), ip(prim), C.byref(flat),
+                                    C.byref(bound), C.byref(deepest))
     assert rc == 0
+    # the kernels size their stacks by this bound (LDS part + overflow area)
+    assert deepest.value <= bound.value, (deepest.value, bound.value)
+    print(f"stack entries used {deepest.value} of bound {bound.value}")
     return dict(t=t, kind=kind, inst=inst, prim=prim), bool(flat.value)
 
 

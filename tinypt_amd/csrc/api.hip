@@ -389,9 +389,8 @@ int check_cfg(const tpt_render_config* c) {
   return TPT_OK;
 }
 
-// Deepest traversal stack any ray can need: one entry per TLAS level, the instance sentinel,
-// one entry per BLAS level, plus one of slack.
-int stack_need(const HostScene& H) { return H.max_tlas_depth + std::max(H.max_blas_depth, H.flat_depth) + 2; }
+// Deepest traversal stack any ray can need in the four-wide trees (HostScene::stack4).
+int stack_need(const HostScene& H) { return H.stack4; }
 
 // Global overflow entries per lane beyond the TPT_WF_STACK entries kept in LDS.
 int needed_gstack(const HostScene& H) {

@@ -93,15 +93,19 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
   }
 }
 
-__global__ void __launch_bounds__(TPT_WF_BLOCK) k_extend(WfParams W, int cur) {
+// OVF: the scene's stack bound exceeds the LDS part, so the traversal stack may overflow to the
+// per-thread global area (W.gstack); otherwise that path is compiled out.  Five waves per SIMD
+// fit without spills (the traversal is VALU-bound; more waves hide its load latency).
+template <bool OVF>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_extend(WfParams W, int cur) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
   qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
   const uint32_t n = qm.n;
   const float4* __restrict__ po = W.ps.o[cur];
   const float4* __restrict__ pd = W.ps.d[cur];
-  LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
-               TPT_WF_STACK};
+  LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
+                     TPT_WF_STACK};
   if (gtid() < TPT_WF_SEGS) {  // clear the queues the shading kernel fills next
     W.ps.counters[TPT_CNT_IDX(cur ? TPT_CNT_Q0 : TPT_CNT_Q1, gtid())] = 0;
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, gtid())] = 0;
@@ -292,6 +296,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
 // Any hit for one shadow queue (0: environment, 1: area light); an unoccluded ray adds its
 // contribution to its path's radiance.  A path has at most one ray per queue, so the
 // read-modify-write of L[id] has a single writer per launch.
+template <bool OVF>
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
@@ -299,8 +304,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) 
   const float4* __restrict__ qo = W.ps.sh_o[queue];
   const float4* __restrict__ qd = W.ps.sh_d[queue];
   const float4* __restrict__ qc = W.ps.sh_c[queue];
-  LaneStack st{s_stack + threadIdx.x, W.gstack ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
-               TPT_WF_STACK};
+  LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
+                     TPT_WF_STACK};
   uint32_t traced = 0;
   for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
     const uint32_t slot = qm.slot(j);
@@ -408,7 +413,8 @@ hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t 
   return hipGetLastError();
 }
 hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_extend, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  if (W.gstack) hipLaunchKernelGGL(k_extend<true>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  else hipLaunchKernelGGL(k_extend<false>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
   return hipGetLastError();
 }
 hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
@@ -420,7 +426,8 @@ hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_
   return hipGetLastError();
 }
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, queue);
+  if (W.gstack) hipLaunchKernelGGL(k_shadow<true>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, queue);
+  else hipLaunchKernelGGL(k_shadow<false>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, queue);
   return hipGetLastError();
 }
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s) {

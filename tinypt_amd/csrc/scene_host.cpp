@@ -253,6 +253,20 @@ static int32_t collapse4(const std::vector<Node2>& n2, int32_t ref, std::vector<
   return me;
 }
 
+// Worst-case traversal stack entries taken inside a Node4 tree: every child of a node hit, so
+// count - 1 far children pushed at each node along the deepest root-to-leaf path.
+static int push_need4(const std::vector<Node4>& n4, int32_t ref) {
+  if (ref < 0 || ref == TPT_REF_EMPTY) return 0;
+  const Node4& n = n4[(size_t)ref];
+  int kids = 0, deepest = 0;
+  for (int i = 0; i < 4; i++) {
+    if (n.child[i] == TPT_REF_EMPTY) continue;
+    kids++;
+    deepest = std::max(deepest, push_need4(n4, n.child[i]));
+  }
+  return std::max(kids - 1, 0) + deepest;
+}
+
 // ------------------------------------------------------------------ TLAS (own design)
 // Full-sweep SAH over item boxes, one item per leaf.  The reference has no top-level tree; it
 // visits every instance and then every sphere for each ray (world.rs:86-118).
@@ -618,6 +632,16 @@ bool HostScene::build(const tpt_scene_desc* d) {
   }
   if (flat) inst32[(size_t)flat_inst].root = collapse4(nodes2, inst32[(size_t)flat_inst].root, nodes4);
   tlas_root = collapse4(nodes2, tlas_root, nodes4);
+  if (nodes4.size() > ((size_t)1 << 25)) {  // the traversal addresses nodes by 32-bit byte offsets
+    error = "scene too large: more than 2^25 four-wide BVH nodes";
+    return false;
+  }
+  // stack bound of the FP32 traversal: top-level pushes, the instance sentinel, the deepest
+  // bottom-level tree's pushes, one entry of slack
+  int blas_push = 0;
+  for (int m = 0; m < nmesh; m++) blas_push = std::max(blas_push, push_need4(nodes4, mesh_root2[(size_t)m]));
+  if (flat) blas_push = std::max(blas_push, push_need4(nodes4, inst32[(size_t)flat_inst].root));
+  stack4 = push_need4(nodes4, tlas_root) + 1 + blas_push + 1;
 
   // ---- environment: EnvMap::from_pixels (env.rs:61-89) + alias table over the same weights
   if (env_kind == TPT_ENV_IMAGE) {

@@ -99,6 +99,7 @@ struct HostScene {
   std::vector<float> cell_pdf;       // per cell: (w/total) * W * H / (2 pi^2), the pdf numerator
   int max_blas_depth = 0, max_tlas_depth = 0;
   int flat_depth = 0;  // levels of the flattened tree
+  int stack4 = 0;      // traversal stack entries the FP32 (four-wide) kernels can need
 
   bool build(const tpt_scene_desc* d);
   void rebuild_lights();

@@ -48,21 +48,24 @@ struct HitF32 {
   int32_t prim;  // reordered triangle slot, -1 for spheres / miss
 };
 
-struct LaneStack {
+// OVF = false: the tree's stack need fits the LDS part, so there is no overflow path.
+template <bool OVF>
+struct LaneStackT {
   int32_t* lds;     // this lane's first LDS entry
   int32_t* glob;    // this lane's global overflow area (may be null if never needed)
   int stride;       // LDS stride in entries (block size)
   int cap;          // LDS entries per lane
   TPT_HD void push(int& sp, int32_t v) const {
-    if (sp < cap) lds[sp * stride] = v;
+    if (!OVF || sp < cap) lds[sp * stride] = v;
     else glob[sp - cap] = v;
     ++sp;
   }
   TPT_HD int32_t pop(int& sp) const {
     --sp;
-    return sp < cap ? lds[sp * stride] : glob[sp - cap];
+    return !OVF || sp < cap ? lds[sp * stride] : glob[sp - cap];
   }
 };
+using LaneStack = LaneStackT<true>;
 
 TPT_HD float safe_rcp(float d) { return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d)); }
 
@@ -95,13 +98,23 @@ TPT_HD bool sphere_hit64(const double* s, vec3<float> of, vec3<float> df, double
   return true;
 }
 
+// Byte offsets of the near planes of a Node4 for a ray direction: lo_x / hi_x at 0 / 16,
+// lo_y / hi_y at 32 / 48, lo_z / hi_z at 64 / 80; the far plane is at near ^ 16.  Along an axis
+// with a positive inverse direction the lower plane gives the smaller distance (the FMA is
+// monotone in the plane coordinate), so loading near and far planes by the ray's signs gives
+// the slab entry and exit distances without a min / max per axis -- bit for bit the same values.
+TPT_HD uint32_t near_off(float inv, uint32_t axis) { return axis * 32u + (inv < 0.f ? 16u : 0u); }
+static_assert(offsetof(Node4, hi_x) == 16 && offsetof(Node4, lo_y) == 32 && offsetof(Node4, hi_y) == 48 &&
+                  offsetof(Node4, lo_z) == 64 && offsetof(Node4, hi_z) == 80 && offsetof(Node4, child) == 96,
+              "near_off assumes the Node4 plane layout");
+
 // Returns true on a hit.  ANY: stop at the first hit in range (shadow rays).
 // Loop shape: an inner loop descends through inner nodes (near child kept in a register, far
 // child pushed); the outer loop then handles the leaf, top-level item or instance exit that
 // the descent stopped at, and pops the next entry.
-template <bool ANY>
+template <bool ANY, class Stack>
 TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmin, float tmax, HitF32* hit,
-                      const LaneStack& st) {
+                      const Stack& st) {
   hit->t = tmax;
   hit->inst = -1;
   hit->prim = -1;
@@ -110,6 +123,8 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
   const vec3<float> inv_w = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
   const vec3<float> oi_w = mk(o.x * inv_w.x, o.y * inv_w.y, o.z * inv_w.z);
   vec3<float> co = o, cd = d, inv = inv_w, oi = oi_w;
+  const uint32_t nx_w = near_off(inv_w.x, 0), ny_w = near_off(inv_w.y, 1), nz_w = near_off(inv_w.z, 2);
+  uint32_t nx = nx_w, ny = ny_w, nz = nz_w;
   float lo = tmin;        // lower t bound at the current level
   float det_eps = 1e-10f;
   float best_t = tmax;
@@ -123,22 +138,30 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
     // is entered, the others are pushed farthest first
     bool have = true;
     while (ref >= 0 && ref != TPT_STACK_SENTINEL) {
-      const float4* p = reinterpret_cast<const float4*>(S.nodes + ref);
-      const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-      const int4 ch = reinterpret_cast<const int4*>(p)[6];
+      // 32-bit byte offsets from the node array (the host keeps it under 4 GiB), so each load
+      // is the array base plus one offset register
+      const char* nb = reinterpret_cast<const char*>(S.nodes);
+      const uint32_t pn = (uint32_t)ref * (uint32_t)sizeof(Node4);
+      const float4 nx4 = *reinterpret_cast<const float4*>(nb + (pn + nx));
+      const float4 fx4 = *reinterpret_cast<const float4*>(nb + (pn + (nx ^ 16u)));
+      const float4 ny4 = *reinterpret_cast<const float4*>(nb + (pn + ny));
+      const float4 fy4 = *reinterpret_cast<const float4*>(nb + (pn + (ny ^ 16u)));
+      const float4 nz4 = *reinterpret_cast<const float4*>(nb + (pn + nz));
+      const float4 fz4 = *reinterpret_cast<const float4*>(nb + (pn + (nz ^ 16u)));
+      const int4 ch = *reinterpret_cast<const int4*>(nb + (pn + 96u));
       float dn[4];
       int32_t cr[4] = {ch.x, ch.y, ch.z, ch.w};
-      const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
+      const float bnx[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, bfx[4] = {fx4.x, fx4.y, fx4.z, fx4.w};
+      const float bny[4] = {ny4.x, ny4.y, ny4.z, ny4.w}, bfy[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
+      const float bnz[4] = {nz4.x, nz4.y, nz4.z, nz4.w}, bfz[4] = {fz4.x, fz4.y, fz4.z, fz4.w};
       int nhit = 0;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const float x0 = fmaf(blx[k], inv.x, -oi.x), x1 = fmaf(bhx[k], inv.x, -oi.x);
-        const float y0 = fmaf(bly[k], inv.y, -oi.y), y1 = fmaf(bhy[k], inv.y, -oi.y);
-        const float z0 = fmaf(blz[k], inv.z, -oi.z), z1 = fmaf(bhz[k], inv.z, -oi.z);
-        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), lo));
-        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best_t)) * 1.0000003f;
+        const float xn = fmaf(bnx[k], inv.x, -oi.x), xf = fmaf(bfx[k], inv.x, -oi.x);
+        const float yn = fmaf(bny[k], inv.y, -oi.y), yf = fmaf(bfy[k], inv.y, -oi.y);
+        const float zn = fmaf(bnz[k], inv.z, -oi.z), zf = fmaf(bfz[k], inv.z, -oi.z);
+        const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, lo));
+        const float tf = fminf(fminf(xf, yf), fminf(zf, best_t)) * 1.0000003f;
         const bool hit = tn <= tf && cr[k] != TPT_REF_EMPTY;
         dn[k] = hit ? tn : INFINITY;
         nhit += hit ? 1 : 0;
@@ -164,6 +187,7 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
       if (ref == TPT_STACK_SENTINEL) {
         // ---- leaving an instance: back to the world ray
         co = o; cd = d; inv = inv_w; oi = oi_w; lo = tmin; det_eps = 1e-10f; cur_inst = -1;
+        nx = nx_w; ny = ny_w; nz = nz_w;
       } else if (cur_inst < 0) {
         // ---- top-level leaf: one item
         const int32_t item = S.tlas_items[(uint32_t)ref & TPT_LEAF_START_MASK];
@@ -202,6 +226,7 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
             det_eps = 1e-10f * len;
             inv = mk(safe_rcp(cd.x), safe_rcp(cd.y), safe_rcp(cd.z));
             oi = mk(co.x * inv.x, co.y * inv.y, co.z * inv.z);
+            nx = near_off(inv.x, 0); ny = near_off(inv.y, 1); nz = near_off(inv.z, 2);
             cur_inst = item;
             cur_key = TPT_KEY_INST(item);
             st.push(sp, TPT_STACK_SENTINEL);
