@@ -516,13 +516,22 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
                      const int32_t* pix_map, double* acc, double* acc_w, double* acc_sq, tpt_stats* st) {
   hipStream_t s = c->stream;
   const uint32_t spp = (uint32_t)cfg->spp;
-  const uint32_t target = 1u << 24;  // paths in flight (about 3.8 GB of path state)
+  // Paths in flight: up to 2^26 (about 18 GB of path state and queues) so that a 512^2 x 256 spp
+  // render is one pass with one drain of the bounce loop; less when the device has less free.
+  uint32_t target = 1u << 26;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && c->ps_cap < target) {
+      const size_t per_path = 280;  // both path sets, hit, L, both shadow queues, segment slack
+      while (target > (1u << 20) && (size_t)target * per_path > free_b / 2) target >>= 1;
+    }
+  }
   uint32_t k = cfg->samples_per_pass > 0 ? (uint32_t)cfg->samples_per_pass : std::max(1u, target / std::max(npix, 1u));
   k = std::max(1u, std::min(k, std::max(spp, 1u)));
   uint32_t pix_chunk = npix;
-  if ((uint64_t)npix * k > (uint64_t)1 << 25) {  // very large images: split the pixels
+  if ((uint64_t)npix * k > (uint64_t)target * 2) {  // very large images: split the pixels
     k = 1;
-    pix_chunk = std::min(npix, 1u << 25);
+    pix_chunk = std::min(npix, target);
   }
   const int gcap = needed_gstack(sc->host);
   size_t P = (size_t)pix_chunk * k;
