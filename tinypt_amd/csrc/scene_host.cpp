@@ -157,6 +157,114 @@ RefBvh build_ref_bvh(const std::vector<DAabb>& prim_bounds, size_t leaf_size) {
   return std::move(b.out);
 }
 
+// ------------------------------------------------------------------ SAH build for the FP32 trees (own design)
+// The FP32 kernels need not walk the reference's tree (the plain FP64 form does): their trees
+// come from a binned SAH build over all three axes with a cost-based leaf decision, which keeps
+// leaves small where that saves triangle tests (the reference splits every node of fewer than
+// 16 primitives at the middle of its input order).  Same RefBvh form, so the FP32 conversion
+// below serves both.
+namespace {
+struct SahBuilder {
+  static constexpr int kBins = 32;
+  static constexpr double kTraverse = 1.0;   // cost of one binary node visit
+  static constexpr double kIntersect = 1.0;  // cost of one triangle test
+  const std::vector<DAabb>& pb;
+  std::vector<double> cen;  // 3 per primitive: box centres
+  size_t max_leaf;
+  RefBvh out;
+  SahBuilder(const std::vector<DAabb>& b, size_t leaf) : pb(b), max_leaf(leaf) {
+    cen.resize(3 * b.size());
+    for (size_t i = 0; i < b.size(); i++)
+      for (int k = 0; k < 3; k++) cen[3 * i + k] = 0.5 * (b[i].mn[k] + b[i].mx[k]);
+  }
+  static double half_area(const DAabb& b) {
+    const double x = b.mx[0] - b.mn[0], y = b.mx[1] - b.mn[1], z = b.mx[2] - b.mn[2];
+    return x * y + y * z + z * x;
+  }
+  int32_t node(size_t lo, size_t hi, int depth) {
+    out.depth = std::max(out.depth, depth);
+    DAabb box = empty_box(), cb = empty_box();
+    for (size_t i = lo; i < hi; i++) {
+      const size_t p = (size_t)out.indices[i];
+      box = unite(box, pb[p]);
+      cb = grow(cb, &cen[3 * p]);
+    }
+    const int32_t me = (int32_t)out.nodes.size();
+    out.nodes.push_back(RefNode{box, -1, -1, (uint32_t)lo, (uint32_t)(hi - lo)});
+    const size_t n = hi - lo;
+    if (n <= 1) return me;
+    // best binned split over the three axes
+    double best = kInf;
+    int best_axis = -1, best_bin = 0;
+    const double parent = half_area(box);
+    for (int axis = 0; axis < 3; axis++) {
+      const double ext = cb.mx[axis] - cb.mn[axis];
+      if (!(ext > 0.0)) continue;
+      DAabb bb[kBins];
+      size_t bc[kBins];
+      for (int b = 0; b < kBins; b++) { bb[b] = empty_box(); bc[b] = 0; }
+      const double scale = kBins / ext;
+      for (size_t i = lo; i < hi; i++) {
+        const size_t p = (size_t)out.indices[i];
+        int b = (int)((cen[3 * p + axis] - cb.mn[axis]) * scale);
+        b = std::min(std::max(b, 0), kBins - 1);
+        bb[b] = unite(bb[b], pb[p]);
+        bc[b]++;
+      }
+      double right_area[kBins];
+      size_t right_n[kBins];
+      DAabb acc = empty_box();
+      size_t cnt = 0;
+      for (int b = kBins - 1; b > 0; b--) {
+        acc = unite(acc, bb[b]);
+        cnt += bc[b];
+        right_area[b] = half_area(acc);
+        right_n[b] = cnt;
+      }
+      acc = empty_box();
+      cnt = 0;
+      for (int b = 1; b < kBins; b++) {
+        acc = unite(acc, bb[b - 1]);
+        cnt += bc[b - 1];
+        if (cnt == 0 || right_n[b] == 0) continue;
+        const double cost = kTraverse + kIntersect * (half_area(acc) * (double)cnt + right_area[b] * (double)right_n[b]) /
+                                            std::max(parent, 1e-300);
+        if (cost < best) { best = cost; best_axis = axis; best_bin = b; }
+      }
+    }
+    const double leaf_cost = kIntersect * (double)n;
+    if (n <= max_leaf && (best_axis < 0 || leaf_cost <= best)) return me;
+    size_t mid;
+    if (best_axis < 0) {  // every centre coincides: split the range in two
+      mid = lo + n / 2;
+    } else {
+      const double scale = kBins / (cb.mx[best_axis] - cb.mn[best_axis]);
+      auto first = out.indices.begin() + (std::ptrdiff_t)lo, last = out.indices.begin() + (std::ptrdiff_t)hi;
+      auto it = std::stable_partition(first, last, [&](int64_t p) {
+        int b = (int)((cen[3 * (size_t)p + best_axis] - cb.mn[best_axis]) * scale);
+        return std::min(std::max(b, 0), kBins - 1) < best_bin;
+      });
+      mid = (size_t)(it - out.indices.begin());
+      if (mid == lo || mid == hi) mid = lo + n / 2;
+    }
+    const int32_t l = node(lo, mid, depth + 1);
+    const int32_t r = node(mid, hi, depth + 1);
+    out.nodes[(size_t)me].left = l;
+    out.nodes[(size_t)me].right = r;
+    out.nodes[(size_t)me].count = 0;
+    return me;
+  }
+};
+}  // namespace
+
+RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf) {
+  SahBuilder b(prim_bounds, max_leaf);
+  b.out.indices.resize(prim_bounds.size());
+  std::iota(b.out.indices.begin(), b.out.indices.end(), 0);
+  if (!prim_bounds.empty()) b.node(0, prim_bounds.size(), 1);
+  return std::move(b.out);
+}
+
 // ------------------------------------------------------------------ FP32 conversion
 static inline float round_down(double x) {
   float f = (float)x;
@@ -425,7 +533,8 @@ bool HostScene::build(const tpt_scene_desc* d) {
     return false;
   }
 
-  // ---- per-mesh BLAS: reference build, reference layout, reordered FP32 triangles, Node2
+  // ---- per-mesh BLAS: reference build and layout (plain FP64 form); own SAH tree, reordered
+  // FP32 triangles and Node2 (FP32 forms)
   blas.resize((size_t)nmesh);
   meshes.resize((size_t)nmesh);
   mesh_root2.assign((size_t)nmesh, TPT_REF_EMPTY);
@@ -452,9 +561,11 @@ bool HostScene::build(const tpt_scene_desc* d) {
       ref_nodes.push_back(x);
     }
     for (int64_t ix : r.indices) ref_indices.push_back((int32_t)ix);
-    // reordered FP32 triangles: slot b0 + k holds triangle indices[k]
-    for (size_t k = 0; k < r.indices.size(); k++) {
-      int64_t t = b0 + r.indices[k];
+    // the FP32 tree (own SAH build) and its reordered triangles: slot b0 + k holds triangle
+    // indices[k] of that tree
+    const RefBvh f = build_sah_bvh(bounds, 4);
+    for (size_t k = 0; k < f.indices.size(); k++) {
+      int64_t t = b0 + f.indices[k];
       const double* v = &tri_v[9 * (size_t)t];
       TriF32& o = tris32[(size_t)(b0 + (int64_t)k)];
       for (int c = 0; c < 3; c++) {
@@ -463,11 +574,11 @@ bool HostScene::build(const tpt_scene_desc* d) {
         o.e2[c] = (float)(v[6 + c] - v[c]);
       }
       o.v0[3] = o.e1[3] = o.e2[3] = 0.f;
-      tri_orig[(size_t)(b0 + (int64_t)k)] = (int32_t)r.indices[k];
+      tri_orig[(size_t)(b0 + (int64_t)k)] = (int32_t)f.indices[k];
       tri_mat32[(size_t)(b0 + (int64_t)k)] = tri_mat[(size_t)t];
     }
-    if (!r.nodes.empty()) {
-      mesh_root2[(size_t)m] = to_node2(r, 0, b0, nodes2, error);
+    if (!f.nodes.empty()) {
+      mesh_root2[(size_t)m] = to_node2(f, 0, b0, nodes2, error);
       if (!error.empty()) return false;
     }
     meshes[(size_t)m] = md;
@@ -559,7 +670,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
       }
       std::vector<DAabb> fb((size_t)total);
       for (int64_t t = 0; t < total; t++) fb[(size_t)t] = triangle_bounds(&wv[9 * (size_t)t]);
-      RefBvh fr = build_ref_bvh(fb, 4);
+      RefBvh fr = build_sah_bvh(fb, 4);
       flat_depth = fr.depth;
       const int64_t base = (int64_t)tris32.size();
       tris32.resize((size_t)(base + total));

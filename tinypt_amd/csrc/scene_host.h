@@ -41,6 +41,10 @@ struct RefBvh {
 // meshes.  Centroids are box centres, as Triangle::centroid (geometry.rs:241-243).
 RefBvh build_ref_bvh(const std::vector<DAabb>& prim_bounds, size_t leaf_size);
 
+// The FP32 kernels' trees: binned SAH over three axes with cost-based leaves of at most
+// `max_leaf` primitives (own design; the node form of RefBvh).
+RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf);
+
 // Triangle::bounds (geometry.rs:221-225) of a static triangle.
 DAabb triangle_bounds(const double* v9);
 
