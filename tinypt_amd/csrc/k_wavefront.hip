@@ -16,6 +16,10 @@
 #include "tpt_trace32.h"
 #include "wavefront.h"
 
+#include <algorithm>
+#include <mutex>
+#include <unordered_map>
+
 namespace tpt {
 
 // ---------------------------------------------------------------- queue compaction
@@ -408,26 +412,57 @@ static inline uint32_t grid_for(uint32_t n) {
   return b < TPT_WF_SEGS ? TPT_WF_SEGS : (b > TPT_WF_MAX_BLOCKS ? TPT_WF_MAX_BLOCKS : b);
 }
 
+// Blocks of `kernel` that are resident on the whole device at once (CUs x blocks per CU at the
+// kernel's register and LDS use), rounded down to a multiple of TPT_WF_SEGS.  A grid-stride
+// kernel launched with more blocks than this runs its last blocks after the first ones finish,
+// with the device partly idle (5 resident blocks per CU against 8 launched leaves a 60 %-full
+// second round); capping the grid at the resident count gives every block the same share.
+static uint32_t resident_blocks(const void* kernel) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, uint32_t> cache;  // one device model per process
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int dev = 0, cus = 0, per_cu = 0;
+  uint32_t r = TPT_WF_MAX_BLOCKS;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TPT_WF_BLOCK, 0) == hipSuccess && cus > 0 &&
+      per_cu > 0) {
+    r = (uint32_t)cus * (uint32_t)per_cu / TPT_WF_SEGS * TPT_WF_SEGS;
+    r = std::max<uint32_t>(TPT_WF_SEGS, std::min<uint32_t>(r, TPT_WF_MAX_BLOCKS));
+  }
+  cache[kernel] = r;
+  return r;
+}
+
+static inline dim3 grid_of(uint32_t n, const void* kernel) { return dim3(std::min(grid_for(n), resident_blocks(kernel))); }
+
 hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s) {
   hipLaunchKernelGGL(k_raygen, dim3(grid_for(A.n_paths)), dim3(TPT_WF_BLOCK), 0, s, W, A);
   return hipGetLastError();
 }
 hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  if (W.gstack) hipLaunchKernelGGL(k_extend<true>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
-  else hipLaunchKernelGGL(k_extend<false>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  if (W.gstack)
+    hipLaunchKernelGGL(k_extend<true>, grid_of(max_n, (const void*)k_extend<true>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  else
+    hipLaunchKernelGGL(k_extend<false>, grid_of(max_n, (const void*)k_extend<false>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
   return hipGetLastError();
 }
 hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  hipLaunchKernelGGL(k_shade, grid_of(max_n, (const void*)k_shade), dim3(TPT_WF_BLOCK), 0, s, W, cur);
   return hipGetLastError();
 }
 hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_light_selfocc, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W);
+  hipLaunchKernelGGL(k_light_selfocc, grid_of(max_n, (const void*)k_light_selfocc), dim3(TPT_WF_BLOCK), 0, s, W);
   return hipGetLastError();
 }
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s) {
-  if (W.gstack) hipLaunchKernelGGL(k_shadow<true>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, queue);
-  else hipLaunchKernelGGL(k_shadow<false>, dim3(grid_for(max_n)), dim3(TPT_WF_BLOCK), 0, s, W, queue);
+  if (W.gstack)
+    hipLaunchKernelGGL(k_shadow<true>, grid_of(max_n, (const void*)k_shadow<true>), dim3(TPT_WF_BLOCK), 0, s, W, queue);
+  else
+    hipLaunchKernelGGL(k_shadow<false>, grid_of(max_n, (const void*)k_shadow<false>), dim3(TPT_WF_BLOCK), 0, s, W,
+                       queue);
   return hipGetLastError();
 }
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s) {
@@ -438,6 +473,7 @@ hipError_t wf_launch_accumulate_adaptive(const WfParams& W, const AdaptiveArgs& 
   hipLaunchKernelGGL(k_accumulate_adaptive, dim3(grid_for(A.n_active)), dim3(TPT_WF_BLOCK), 0, s, W, A);
   return hipGetLastError();
 }
+// Upper bound of the threads of any launch above over n entries (sizes per-thread areas).
 uint32_t wf_grid_threads(uint32_t n) { return grid_for(n) * TPT_WF_BLOCK; }
 
 }  // namespace tpt
