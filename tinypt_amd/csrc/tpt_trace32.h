@@ -170,6 +170,22 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
         have = false;
         break;
       }
+      if (ANY) {
+        // any hit ends the ray, so the visiting order does not matter: enter the first hit
+        // child, push the others unsorted
+        int32_t nref = TPT_REF_EMPTY;
+        bool got = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (dn[k] != INFINITY) {
+            if (got) st.push(sp, cr[k]);
+            else nref = cr[k];
+            got = true;
+          }
+        }
+        ref = nref;
+        continue;
+      }
       // sort (distance, child) ascending: 5-comparator network; misses sort last
 #define TPT_CSWAP(a, b)                                              \
   if (dn[b] < dn[a]) {                                               \
