@@ -410,8 +410,8 @@ int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
     }
     HIPC(c->ps_hit.ensure(P * 16));
     HIPC(c->ps_L.ensure(P * 16));
+    HIPC(c->sh_o[0].ensure(Q * 16));  // the light queue keeps its origin in sh_l
     for (int q = 0; q < 2; q++) {
-      HIPC(c->sh_o[q].ensure(Q * 16));
       HIPC(c->sh_d[q].ensure(Q * 16));
       HIPC(c->sh_c[q].ensure(Q * 16));
     }
@@ -443,7 +443,7 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   ps.hit = c->ps_hit.as<int4>();
   ps.L = c->ps_L.as<float4>();
   for (int q = 0; q < 2; q++) {
-    ps.sh_o[q] = c->sh_o[q].as<float4>();
+    ps.sh_o[q] = q == 0 ? c->sh_o[0].as<float4>() : nullptr;
     ps.sh_d[q] = c->sh_d[q].as<float4>();
     ps.sh_c[q] = c->sh_c[q].as<float4>();
   }

@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     const bool last_nd = (flags & 0x100u) != 0;
     // ---- phase 1: miss, Russian roulette, emission, next-event estimation
     float4 sh_c0 = make_float4(0.f, 0.f, 0.f, 0.f), sh_c1 = sh_c0;
-    float4 sh_o0 = sh_c0, sh_d0 = sh_c0, sh_o1 = sh_c0, sh_d1 = sh_c0, sh_l = sh_c0;  // 0: environment, 1: light
+    float4 sh_o0 = sh_c0, sh_d0 = sh_c0, sh_d1 = sh_c0, sh_l = sh_c0;  // 0: environment, 1: light
     float2 sh_lv = make_float2(0.f, 0.f);
     bool go = false;  // hit a scattering surface: sample the BSDF in phase 2
     f3 p = mk(0.f, 0.f, 0.f), ng = p;
@@ -229,11 +229,10 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
                   bsdf_eval(kind, mc, mp.w, wo, wi, n_o, &f, &pdf_b);
                   f3 c = had(had(beta, f), ls.emit) * (cs * mis_weight(ls.pdf, pdf_b) / ls.pdf);
                   if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
-                    f3 so = p + (float)TPT_RAY_EPSILON * wi;
-                    sh_o1 = make_float4(so.x, so.y, so.z, light_shadow_tmax32(dist));
+                    // the shadow kernel rebuilds origin p + eps wi and tmax from p and dist
                     sh_d1 = make_float4(wi.x, wi.y, wi.z, __int_as_float(id));
-                    sh_c1 = make_float4(c.x, c.y, c.z, 1.f);
-                    sh_l = make_float4(p.x, p.y, p.z, ls.u);  // for k_light_selfocc
+                    sh_c1 = make_float4(c.x, c.y, c.z, dist);
+                    sh_l = make_float4(p.x, p.y, p.z, ls.u);  // also for k_light_selfocc
                     sh_lv = make_float2(ls.v, __int_as_float(ls.idx));
                     has_light = true;
                   }
@@ -252,7 +251,6 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     }
     const uint32_t jl = seg_slot(has_light, W.ps, TPT_CNT_SHL, s_slot);
     if (has_light && jl != UINT32_MAX) {
-      W.ps.sh_o[1][jl] = sh_o1;
       W.ps.sh_d[1][jl] = sh_d1;
       W.ps.sh_c[1][jl] = sh_c1;
       W.ps.sh_l[jl] = sh_l;
@@ -293,7 +291,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
     float4 l = W.ps.sh_l[slot];
     float2 lv = W.ps.sh_lv[slot];
     if (light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x))
-      W.ps.sh_c[1][slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+      W.ps.sh_c[1][slot] = make_float4(0.f, 0.f, 0.f, -1.f);
   }
 }
 
@@ -305,7 +303,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) 
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
   qm.load(W.ps.counters, queue ? TPT_CNT_SHL : TPT_CNT_SH, W.ps.seg_cap);
-  const float4* __restrict__ qo = W.ps.sh_o[queue];
+  const float4* __restrict__ qo = queue ? W.ps.sh_l : W.ps.sh_o[0];
   const float4* __restrict__ qd = W.ps.sh_d[queue];
   const float4* __restrict__ qc = W.ps.sh_c[queue];
   LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
@@ -314,11 +312,17 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) 
   for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
     const uint32_t slot = qm.slot(j);
     const float4 c = qc[slot];
-    if (c.w == 0.f) continue;  // dropped (self-occluded light sample)
+    if (!(c.w > 0.f)) continue;  // dropped (self-occluded light sample)
     const float4 o4 = qo[slot], d4 = qd[slot];
+    f3 o = f3of(o4);
+    float tmax = o4.w;
+    if (queue) {  // light queue: o4 is the shading point, c.w the distance to the light sample
+      o = o + (float)TPT_RAY_EPSILON * f3of(d4);
+      tmax = light_shadow_tmax32(c.w);
+    }
     HitF32 h;
     traced++;
-    if (!trace_f32<true>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, o4.w, &h, st))
+    if (!trace_f32<true>(W.S, o, f3of(d4), (float)TPT_RAY_EPSILON, tmax, &h, st))
       add_radiance(W.ps.L, __float_as_int(d4.w), f3of(c));
   }
   // one atomic per wave for the shadow-ray count

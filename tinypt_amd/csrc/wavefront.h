@@ -55,10 +55,11 @@ struct PathState {
   uint64_t* rng[2];
   int4* hit;        // closest hit of queue position i: t bits, instance code, triangle slot
   float4* L;        // radiance gathered so far, by path id
-  // shadow rays, one dense queue per kind: [0] environment, [1] area light
-  float4* sh_o[2];  // origin, tmax
+  // shadow rays, one segmented queue per kind: [0] environment, [1] area light
+  float4* sh_o[2];  // environment queue: origin, tmax ([1] unused)
   float4* sh_d[2];  // direction, path id (int bits)
-  float4* sh_c[2];  // contribution if unoccluded; w = 1, set to 0 when the ray is dropped
+  float4* sh_c[2];  // contribution if unoccluded; w: 1 (environment) or the distance to the light
+                    // sample (light; origin p + eps wi and tmax follow from it); w <= 0: dropped
   float4* sh_l;     // light queue: shading point xyz, light-sample u
   float2* sh_lv;    // light queue: light-sample v, light index (int bits)
   uint32_t* counters;
