@@ -142,3 +142,12 @@ def test_write_rendered_image(cornell, tmp_path):
         p = str(tmp_path / name)
         O.write_image(p, out.acc, out.acc_w, exposure=cfg.exposure, tonemap=cfg.tonemap)
         assert os.path.getsize(p) > 0
+
+
+def test_progress_render_equals_full(cornell, capsys):
+    cfg = _cfg()
+    full = T.render(cornell, cfg)
+    out = T.render(cornell, cfg, progress=True)
+    err = capsys.readouterr().err
+    assert "Progress: 15/15 tiles (100.0%)" in err and "Done in" in err
+    assert np.array_equal(out.acc, full.acc) and np.array_equal(out.acc_w, full.acc_w)

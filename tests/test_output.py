@@ -349,3 +349,27 @@ def test_checkpointed_loop_merges_and_saves(tmp_path, monkeypatch):
     os.remove(ck)
     out = T.render(object.__new__(T.DeviceScene), cfg, ckpt_file=ck)
     assert len(calls) == 1 and out.acc_sq is not None
+
+
+def test_progress_lines(capsys, monkeypatch):
+    """render.rs:101-105, 216-263: the Render line, Progress lines ending at 100%, Done."""
+    w, h = 40, 24
+    cfg = T.RenderConfig(width=w, height=h, spp=2, tile=8, seed=7)
+    calls = []
+    monkeypatch.setattr(T, "_TILE_BATCH_PATHS", 200)  # several batches
+    monkeypatch.setattr(T, "render_pixels", _fake_render_pixels(calls))
+
+    class _Ctx:
+        device = 0
+
+    ds = object.__new__(T.DeviceScene)
+    ds.ctx = _Ctx()
+    out = T.render(ds, cfg, progress=True)
+    err = capsys.readouterr().err
+    assert err.startswith("Render: 40x24, spp=2, tile=8, device=hip:0, morton=true, seed=7\n")
+    assert "\rProgress: 0/15 tiles (0.0%)" in err
+    assert "\rProgress: 15/15 tiles (100.0%)\n" in err
+    assert err.rstrip().splitlines()[-1].startswith("Done in ")
+    assert len(calls) > 1 and np.array_equal(out.acc_w, np.full((h, w), 2.0))
+    idx = np.arange(w * h, dtype=np.float64).reshape(h, w)
+    assert np.array_equal(out.acc[..., 2], 3 * idx)

@@ -239,7 +239,7 @@ class RenderOutput:
 
 
 def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Context] = None,
-           ckpt_file: Optional[str] = None) -> RenderOutput:
+           ckpt_file: Optional[str] = None, progress: bool = False) -> RenderOutput:
     """render() (render.rs:42-278) on the GPU.  ``scene`` is a Scene or an uploaded
     DeviceScene.
 
@@ -248,10 +248,13 @@ def render(scene, config: RenderConfig, want_sq: bool = True, ctx: Optional[Cont
     the remaining tiles in task order (batches of whole checkpoint intervals, one GPU call per
     batch), merges them in task-id order and saves the HYPCKPT file every
     ``checkpoint_every_tasks`` tasks and at the end.  ``acc_sq`` is then None when the render
-    resumed (the file holds no squares).  Otherwise the whole image is one GPU call."""
+    resumed (the file holds no squares).  ``progress`` prints the reference's stderr lines
+    (``Render: ...``, ``Progress: n/N tiles (p%)`` at most every 250 ms, ``Done in ...``) and
+    also renders tile batches.  Otherwise the whole image is one GPU call."""
     dscene = scene if isinstance(scene, DeviceScene) else DeviceScene(scene, ctx)
-    if ckpt_file is not None and config.checkpoint_enabled and config.checkpoint_every_tasks > 0:
-        return _render_checkpointed(dscene, config, ckpt_file, want_sq)
+    ckpt_on = ckpt_file is not None and config.checkpoint_enabled and config.checkpoint_every_tasks > 0
+    if ckpt_on or progress:
+        return _render_tiled(dscene, config, ckpt_file if ckpt_on else None, want_sq, progress)
     lib = dscene._lib
     h, w = int(config.height), int(config.width)
     acc = np.zeros((h, w, 3))
@@ -281,39 +284,54 @@ def render_pixels(scene: "DeviceScene", config: RenderConfig, pixels: np.ndarray
 
 
 _CKPT_BATCH_PATHS = 1 << 22  # samples per GPU call below which more checkpoint intervals join a batch
+_TILE_BATCH_PATHS = 1 << 24  # samples per GPU call of a tiled render without checkpoints
+_PROGRESS_INTERVAL_S = 0.25  # constants.rs:30
 
 
-def _render_checkpointed(dscene: "DeviceScene", config: RenderConfig, ckpt_file: str, want_sq: bool) -> RenderOutput:
-    """render.rs:48-278 with checkpoints: tiles in task-id order, merged in that order."""
+def _render_tiled(dscene: "DeviceScene", config: RenderConfig, ckpt_file: Optional[str], want_sq: bool,
+                  progress: bool) -> RenderOutput:
+    """render.rs:48-278 over tile tasks: tiles in task-id order (Morton order when enabled),
+    rendered in batches by tpt_render_pixels, merged in task-id order; with ``ckpt_file`` the
+    render resumes from and saves HYPCKPT checkpoints, with ``progress`` it reports progress."""
     import sys
+    import time
     from .output import load_checkpoint, save_checkpoint, task_pixels, tile_tasks
     h, w = int(config.height), int(config.width)
     acc = np.zeros((h, w, 3))
     acc_w = np.zeros((h, w))
     resume = 0
-    try:
-        got = load_checkpoint(ckpt_file, config.scene_hash, w, h)
-    except TptError:
-        got = None  # render.rs:57 takes only Ok(Some(..))
-    if got is not None:
-        resume, acc, acc_w = got
-        print(f"Resumed from checkpoint: {ckpt_file} (next task id: {resume})", file=sys.stderr)
+    if ckpt_file is not None:
+        try:
+            got = load_checkpoint(ckpt_file, config.scene_hash, w, h)
+        except TptError:
+            got = None  # render.rs:57 takes only Ok(Some(..))
+        if got is not None:
+            resume, acc, acc_w = got
+            print(f"Resumed from checkpoint: {ckpt_file} (next task id: {resume})", file=sys.stderr)
     acc_sq = np.zeros((h, w, 3)) if want_sq and resume == 0 else None
     tasks = tile_tasks(w, h, int(config.tile), bool(config.morton_enabled))
     tid = len(tasks)
-    every = int(config.checkpoint_every_tasks)
+    every = int(config.checkpoint_every_tasks) if ckpt_file is not None else 1
+    batch_paths = _CKPT_BATCH_PATHS if ckpt_file is not None else _TILE_BATCH_PATHS
     flat_acc, flat_w = acc.reshape(-1, 3), acc_w.reshape(-1)
     flat_sq = acc_sq.reshape(-1, 3) if acc_sq is not None else None
     stats = {}
     next_id = min(resume, tid)
     sizes = [(x1 - x0) * (y1 - y0) for x0, y0, x1, y1 in tasks]
+    if progress:  # render.rs:101-105, 216-222
+        print(f"Render: {w}x{h}, spp={config.spp}, tile={config.tile}, device=hip:{dscene.ctx.device}, "
+              f"morton={str(bool(config.morton_enabled)).lower()}, seed={config.seed}", file=sys.stderr)
+        if tid > 0:
+            print(f"\rProgress: {next_id}/{tid} tiles ({next_id * 100.0 / tid:.1f}%)", end="", file=sys.stderr,
+                  flush=True)
+    start = last_print = time.monotonic()
     while next_id < tid:
         end, n_pix = next_id, 0
         while end < tid:
             stop = min(tid, (end // every + 1) * every)
             n_pix += sum(sizes[end:stop])
             end = stop
-            if n_pix * max(int(config.spp), 1) >= _CKPT_BATCH_PATHS:
+            if n_pix * max(int(config.spp), 1) >= batch_paths:
                 break
         pix = task_pixels(tasks[next_id:end], w)
         s, sw, sq, st = render_pixels(dscene, config, pix, want_sq=flat_sq is not None)
@@ -328,10 +346,19 @@ def _render_checkpointed(dscene: "DeviceScene", config: RenderConfig, ckpt_file:
             if flat_sq is not None:
                 flat_sq[p] += sq[sl]
             off += sizes[t]
-            if (t + 1) % every == 0:
+            now = time.monotonic()
+            if progress and (now - last_print >= _PROGRESS_INTERVAL_S or t + 1 == tid):  # render.rs:244-251
+                print(f"\rProgress: {t + 1}/{tid} tiles ({(t + 1) * 100.0 / tid:.1f}%)", end="", file=sys.stderr,
+                      flush=True)
+                last_print = now
+            if ckpt_file is not None and (t + 1) % every == 0:
                 save_checkpoint(ckpt_file, config.scene_hash, t + 1, acc, acc_w)
         next_id = end
-    save_checkpoint(ckpt_file, config.scene_hash, tid, acc, acc_w)  # render.rs:267-272
+    if progress:
+        print(file=sys.stderr)
+        print(f"Done in {time.monotonic() - start:.2f}s", file=sys.stderr)
+    if ckpt_file is not None:
+        save_checkpoint(ckpt_file, config.scene_hash, tid, acc, acc_w)  # render.rs:267-272
     return RenderOutput(acc, acc_w, acc_sq, stats)
 
 
