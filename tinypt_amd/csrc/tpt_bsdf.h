@@ -81,8 +81,8 @@ __device__ __forceinline__ vec3<T> sample_cosine_hemisphere(vec3<T> n, R& rng) {
   T v = rng.uni();
   T r = sqrt(u);
   T phi = T(2) * T(TPT_PI) * v;
-  T x = r * cos(phi);
-  T y = r * sin(phi);
+  T x = r * sample_cos(phi);
+  T y = r * sample_sin(phi);
   T z = sqrt(T(1) - u);
   vec3<T> t, b;
   tangent_frame(n, &t, &b);
@@ -101,8 +101,8 @@ __device__ __forceinline__ vec3<T> sample_ggx_vndf(vec3<T> wo, T alpha, T u1, T 
   }
   T r = sqrt(u1);
   T phi = T(TPT_TAU) * u2;
-  T t1p = r * cos(phi);
-  T t2p_init = r * sin(phi);
+  T t1p = r * sample_cos(phi);
+  T t2p_init = r * sample_sin(phi);
   T s = T(0.5) * (T(1) + v.z);
   T t2p = sqrt(T(1) - s) * t2p_init + s * sqrt(fmax(T(1) - t1p * t1p, T(0)));
   vec3<T> nh = vnorm(t1 * t1p + t2 * t2p + v * sqrt(fmax(T(1) - t1p * t1p - t2p * t2p, T(0))));

@@ -62,7 +62,7 @@ __device__ __forceinline__ void env_sample32(const EnvDev& E, RngF32& rng, f3* d
     float z = 1.f - 2.f * u;
     float r = sqrtf(fmaxf(1.f - z * z, 0.f));
     float phi = (float)TPT_TAU * v;
-    *dir = mk(r * cosf(phi), z, r * sinf(phi));
+    *dir = mk(r * sample_cos(phi), z, r * sample_sin(phi));
     *li = env_lookup32(E, *dir);
     *pdf = 1.0f / (4.0f * (float)TPT_PI);
     return;
@@ -101,7 +101,7 @@ __device__ __forceinline__ bool sample_light32(const LightsDev& Ls, RngF32& rng,
     float z = 1.f - 2.f * u;
     float rr = sqrtf(fmaxf(1.f - z * z, 0.f));
     float phi = (float)TPT_TAU * v;
-    f3 dir = mk(rr * cosf(phi), z, rr * sinf(phi));
+    f3 dir = mk(rr * sample_cos(phi), z, rr * sample_sin(phi));
     pos = ld3(L.p0) + dir * L.p0[3];
     nrm = dir;
     area = 4.f * (float)TPT_PI * L.p0[3] * L.p0[3];
