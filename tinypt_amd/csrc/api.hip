@@ -46,7 +46,7 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);  // release path: nothing to report to
     p = nullptr;
     bytes = 0;
   }
@@ -154,9 +154,9 @@ int tpt_ctx_create(int device, tpt_ctx** out) {
 
 int tpt_ctx_destroy(tpt_ctx* ctx) {
   if (!ctx) return TPT_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  hipStreamDestroy(ctx->stream);
+  (void)hipSetDevice(ctx->device);  // teardown: errors cannot be acted on
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return TPT_OK;
 }
@@ -272,8 +272,8 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
 int tpt_scene_destroy(tpt_scene* scene) {
   if (!scene) return TPT_OK;
   if (scene->ctx) {
-    hipSetDevice(scene->ctx->device);
-    hipStreamSynchronize(scene->ctx->stream);
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
   }
   delete scene;
   return TPT_OK;
@@ -470,7 +470,7 @@ struct EventTimer {
   std::vector<hipEvent_t> ev;
   EventTimer(hipStream_t st, bool enable) : s(st), on(enable) {}
   ~EventTimer() {
-    for (hipEvent_t e : ev) hipEventDestroy(e);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
   hipError_t mark() {
     if (!on) return hipSuccess;

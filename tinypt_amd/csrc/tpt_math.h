@@ -56,22 +56,22 @@ template <class T> TPT_HD vec3<T> ld3(const T* p) { return mk(p[0], p[1], p[2]);
 // (absolute error near 1e-6 over [0, 2 pi), far below the Monte Carlo noise, and the forward and
 // adjoint kernels call the same code); FP64 keeps the libm-equivalent functions of the
 // reference's arithmetic.
-TPT_HD inline float sample_sin(float x) {
+TPT_HD float sample_sin(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __sinf(x);
 #else
   return sinf(x);
 #endif
 }
-TPT_HD inline float sample_cos(float x) {
+TPT_HD float sample_cos(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __cosf(x);
 #else
   return cosf(x);
 #endif
 }
-TPT_HD inline double sample_sin(double x) { return sin(x); }
-TPT_HD inline double sample_cos(double x) { return cos(x); }
+TPT_HD double sample_sin(double x) { return sin(x); }
+TPT_HD double sample_cos(double x) { return cos(x); }
 
 // ---------------------------------------------------------------- RNG (rng.rs:9-47)
 struct Pcg32 {
