@@ -130,6 +130,9 @@ __device__ __forceinline__ void add_radiance(float4* L, int32_t id, f3 c) {
   L[id] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
 }
 
+// ENV_NEE: the environment is an image with a non-zero texel, so environment NEE runs (the
+// variant without it needs fewer registers).
+template <bool ENV_NEE>
 __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shade(WfParams W, int cur) {
   __shared__ uint32_t s_slot[8];
   QueueMap qm;
@@ -197,7 +200,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
             f3 wo = vnorm(-d);
             // nee_environment (integrator.rs:148-172); a black map contributes nothing, so the
             // FP32 form skips it (its own random streams need not follow the reference's draws)
-            if (W.env.kind == TPT_ENV_IMAGE && !W.env.black) {
+            if (ENV_NEE) {
               f3 wi, li;
               float pdf_env;
               env_sample32(W.env, rng, &wi, &li, &pdf_env);
@@ -243,8 +246,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
         }
       }
     }
-    const uint32_t je = seg_slot(has_env, W.ps, TPT_CNT_SH, s_slot);
-    if (has_env && je != UINT32_MAX) {
+    const uint32_t je = ENV_NEE ? seg_slot(has_env, W.ps, TPT_CNT_SH, s_slot) : UINT32_MAX;
+    if (ENV_NEE && has_env && je != UINT32_MAX) {
       W.ps.sh_o[0][je] = sh_o0;
       W.ps.sh_d[0][je] = sh_d0;
       W.ps.sh_c[0][je] = sh_c0;
@@ -454,7 +457,10 @@ hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStrea
   return hipGetLastError();
 }
 hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, grid_of(max_n, (const void*)k_shade), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  if (W.env.kind == TPT_ENV_IMAGE && !W.env.black)
+    hipLaunchKernelGGL(k_shade<true>, grid_of(max_n, (const void*)k_shade<true>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+  else
+    hipLaunchKernelGGL(k_shade<false>, grid_of(max_n, (const void*)k_shade<false>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
   return hipGetLastError();
 }
 hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s) {
