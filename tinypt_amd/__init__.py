@@ -51,6 +51,13 @@ def load_library():
     """Load the built C-ABI library (never builds; run tinypt_amd.build first)."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (soname libamdhip64.so.7).
+        # Loaded first, it is the runtime this library binds to; loaded after this library it
+        # would be a second runtime, and PyTorch would then find no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(_LIB_PATH):
             raise TptError(_abi.TPT_ERR_INTERNAL, f"{_LIB_PATH} is not built (python -m tinypt_amd.build)")
         L = C.CDLL(_LIB_PATH)
