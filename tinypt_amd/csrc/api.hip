@@ -535,7 +535,8 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   }
   const int gcap = needed_gstack(sc->host);
   size_t P = (size_t)pix_chunk * k;
-  if (cfg->adaptive_enabled) P = std::max(P, (size_t)std::min<uint64_t>((uint64_t)npix * 64, (uint64_t)1 << 23));
+  // adaptive passes give each active pixel up to 64 samples, within the same in-flight budget
+  if (cfg->adaptive_enabled) P = std::max(P, (size_t)std::min<uint64_t>((uint64_t)npix * 64, (uint64_t)target));
   int rc = ensure_paths(c, P, gcap);
   if (rc) return rc;
   WfParams W = make_wf(c, sc, cfg, gcap);
