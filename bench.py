@@ -200,12 +200,15 @@ def main():
         # 1024 spp; one step = forward render + path-replay adjoint + Adam update
         from tinypt_amd.inverse import InverseProblem, optimize
         prob = InverseProblem(ctx=ds.ctx)
-        hist = optimize(prob, steps=args.inverse_steps, lr=0.02)
+        hist = optimize(prob, steps=args.inverse_steps, lr=0.02, eval_spp=512)
         inverse = {"s_per_iter": hist["s_per_iter"], "steps": args.inverse_steps, "loss_first": hist["loss"][0],
                    "loss_last10_mean": float(np.mean(hist["loss"][-10:])),
                    "param_error_init": hist["init_error"], "param_error_final": hist["param_error"][-1],
                    "f0_error_init": hist["init_f0_error"], "f0_error_final": hist["f0_error"][-1],
                    "alpha_error_init": hist["init_alpha_error"], "alpha_error_final": hist["alpha_error"][-1],
+                   "image_error_init": hist["image_error_init"], "image_error_final": hist["image_error_final"],
+                   "image_error_truth": hist["image_error_truth"],
+                   "image_error_note": "MSE vs the target of a 512-spp render; the truth value is the noise floor",
                    "config": "cornell + 16 GGX spheres, 256x256, 16 spp, depth 8, target 1024 spp, Adam lr 0.02"}
         log(f"inverse: {inverse}")
 

@@ -71,9 +71,22 @@ class InverseProblem:
         return t.index_put((idx,), sphere_params)
 
 
-def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: int = 1, log=None) -> dict:
+def image_error(prob: InverseProblem, params: np.ndarray, spp: int = 512, seed: int = 0x51CE) -> float:
+    """Mean squared difference between the target and a render of the sphere parameters at
+    ``spp`` (the two images' own Monte Carlo variance included: the value at the true parameters
+    is the floor).  The 16-spp training loss is dominated by that variance; this measures the
+    fit."""
+    prob.dscene.update_materials(prob.table(torch.as_tensor(np.asarray(params, dtype=np.float64))).numpy())
+    out = render(prob.dscene, prob.config(spp, seed), want_sq=False)
+    prob.dscene.update_materials(prob.base)
+    return float(((resolve_pixels(out.acc, out.acc_w) - prob.target.numpy()) ** 2).mean())
+
+
+def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: int = 1, log=None,
+             eval_spp: int = 0) -> dict:
     """Adam on the L2 image loss.  Returns per-step loss and mean absolute parameter error, and
-    the mean wall time of a step (forward render + adjoint + update)."""
+    the mean wall time of a step (forward render + adjoint + update).  With ``eval_spp`` > 0 also
+    the image error (``image_error``) of the initial, final and true parameters."""
     x = torch.tensor(prob.init, dtype=torch.float64, requires_grad=True)
     opt = torch.optim.Adam([x], lr=lr)
     losses, errs, f0_errs, alpha_errs, times = [], [], [], [], []
@@ -100,7 +113,12 @@ def optimize(prob: InverseProblem, steps: int = 200, lr: float = 0.02, seed0: in
             log(f"step {it}: loss {losses[-1]:.6g} param error {errs[-1]:.4f} ({times[-1] * 1e3:.1f} ms)")
     prob.dscene.update_materials(prob.base)
     init_err = np.abs(prob.init - prob.truth)
-    return {"loss": losses, "param_error": errs, "f0_error": f0_errs, "alpha_error": alpha_errs,
+    img = {}
+    if eval_spp > 0:
+        img = {"image_error_init": image_error(prob, prob.init, eval_spp),
+               "image_error_final": image_error(prob, x.detach().numpy(), eval_spp),
+               "image_error_truth": image_error(prob, prob.truth, eval_spp), "image_error_spp": eval_spp}
+    return {**img, "loss": losses, "param_error": errs, "f0_error": f0_errs, "alpha_error": alpha_errs,
             "s_per_iter": float(np.mean(times[1:] if steps > 1 else times)), "params": x.detach().numpy().copy(),
             "init_error": float(init_err.mean()), "init_f0_error": float(init_err[:, :3].mean()),
             "init_alpha_error": float(init_err[:, 3].mean())}

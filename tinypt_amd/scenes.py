@@ -322,10 +322,16 @@ def inverse_scene(params: np.ndarray, width: int = 256, height: int = 256, n_sph
             b.world.meshes[k].mat_ids[:] = k
     k = int(math.ceil(math.sqrt(n_spheres)))
     for i in range(n_spheres):
-        gx, gz = i % k, i // k
+        gx, gy = i % k, i // k
         x = -0.75 + 1.5 * (gx + 0.5) / k
-        z = -0.75 + 1.5 * (gz + 0.5) / k
-        y = 0.13 + 0.45 * ((i * 7) % 5) / 4.0 if with_cubes else radius + 0.01
+        if with_cubes:
+            # a vertical grid in front of the boxes (z = 0.8, above the short box's top at
+            # y = 0.6): every sphere in view, none touching a box
+            y = 0.75 + 0.9 * gy / max(k - 1, 1)
+            z = 0.8
+        else:  # on the floor
+            y = radius + 0.01
+            z = -0.75 + 1.5 * (gy + 0.5) / k
         b.sphere((x, y, z), radius, Ggx(tuple(params[i, :3]), float(params[i, 3])))
     return b.build(cornell_camera(width, height), EnvMap.constant((0.0, 0.0, 0.0)))
 
