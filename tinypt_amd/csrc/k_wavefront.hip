@@ -174,11 +174,13 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     float4 mp = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < n) {
       if (h4.y == -1) {
-        // miss (integrator.rs:71-83)
-        f3 bg = W.env.black ? mk(0.f, 0.f, 0.f) : (W.env.kind == TPT_ENV_IMAGE ? env_lookup32(W.env, d) : sky32(d));
-        f3 c = had(beta, bg);
-        if (W.env.kind == TPT_ENV_IMAGE && last_nd) c = c * mis_weight(last_pdf, env_pdf32(W.env, d));
-        add_radiance(W.ps.L, id, clamp_luminance(c, (float)TPT_FIREFLY_CLAMP));
+        // miss (integrator.rs:71-83); a black map adds nothing, so its paths just end
+        if (!W.env.black) {
+          f3 bg = W.env.kind == TPT_ENV_IMAGE ? env_lookup32(W.env, d) : sky32(d);
+          f3 c = had(beta, bg);
+          if (W.env.kind == TPT_ENV_IMAGE && last_nd) c = c * mis_weight(last_pdf, env_pdf32(W.env, d));
+          add_radiance(W.ps.L, id, clamp_luminance(c, (float)TPT_FIREFLY_CLAMP));
+        }
       } else {
         go = true;
         if (bounce >= W.rr_start) {  // integrator.rs:89-95
