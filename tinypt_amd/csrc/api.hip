@@ -74,7 +74,7 @@ struct tpt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   // wavefront scratch (grow-only)
-  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o[2], sh_d[2], sh_c[2], sh_l, sh_lv, counters, stats,
+  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o[2], sh_d[2], sh_c[2], sh_l, sh_lv, shl_keep, counters, stats,
       gstack;
   // adaptive / plain per-pixel state
   DevBuf act0, act1, nsamp, welford, rngpix, done;
@@ -417,6 +417,7 @@ int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
     }
     HIPC(c->sh_l.ensure(Q * 16));
     HIPC(c->sh_lv.ensure(Q * 8));
+    HIPC(c->shl_keep.ensure(Q * 4));
     c->ps_cap = P;
   }
   HIPC(c->counters.ensure(TPT_NUM_CNT * 4));
@@ -449,6 +450,7 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   }
   ps.sh_l = c->sh_l.as<float4>();
   ps.sh_lv = c->sh_lv.as<float2>();
+  ps.shl_keep = c->shl_keep.as<uint32_t>();
   ps.counters = c->counters.as<uint32_t>();
   ps.stats = c->stats.as<uint64_t>();
   ps.seg_cap = wf_seg_cap((uint32_t)c->ps_cap);

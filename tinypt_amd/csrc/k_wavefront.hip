@@ -94,6 +94,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs 
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_Q1, g)] = 0;
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, g)] = 0;
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, g)] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHK, g)] = 0;
   }
 }
 
@@ -114,6 +115,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     W.ps.counters[TPT_CNT_IDX(cur ? TPT_CNT_Q0 : TPT_CNT_Q1, gtid())] = 0;
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, gtid())] = 0;
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, gtid())] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHK, gtid())] = 0;
   }
   if (gtid() == 0) atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
   for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
@@ -287,16 +289,25 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
 
 // The reference's light shadow ray can be blocked by the light itself; that decision is taken
 // in FP64 (tpt_selfocc64.h) here, in a kernel of its own so the FP64 registers do not weigh on
-// k_shade.  A self-occluded ray is dropped before k_shadow traces it.
+// k_shade.  The slots of the rays that remain go densely into the SHK queue, so k_shadow's
+// waves carry no lanes for dropped rays (about 30% on a rectangle light).
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
+  __shared__ uint32_t s_slot[8];
   QueueMap qm;
   qm.load(W.ps.counters, TPT_CNT_SHL, W.ps.seg_cap);
-  for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
-    const uint32_t slot = qm.slot(j);
-    float4 l = W.ps.sh_l[slot];
-    float2 lv = W.ps.sh_lv[slot];
-    if (light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x))
-      W.ps.sh_c[1][slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+  const uint32_t n = qm.n;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t j = base + threadIdx.x;
+    uint32_t slot = 0;
+    bool keep = false;
+    if (j < n) {
+      slot = qm.slot(j);
+      const float4 l = W.ps.sh_l[slot];
+      const float2 lv = W.ps.sh_lv[slot];
+      keep = !light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x);
+    }
+    const uint32_t k = seg_slot(keep, W.ps, TPT_CNT_SHK, s_slot);  // every thread of the block
+    if (keep && k != UINT32_MAX) W.ps.shl_keep[k] = slot;
   }
 }
 
@@ -307,7 +318,7 @@ template <bool OVF>
 __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
-  qm.load(W.ps.counters, queue ? TPT_CNT_SHL : TPT_CNT_SH, W.ps.seg_cap);
+  qm.load(W.ps.counters, queue ? TPT_CNT_SHK : TPT_CNT_SH, W.ps.seg_cap);
   const float4* __restrict__ qo = queue ? W.ps.sh_l : W.ps.sh_o[0];
   const float4* __restrict__ qd = W.ps.sh_d[queue];
   const float4* __restrict__ qc = W.ps.sh_c[queue];
@@ -315,9 +326,8 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) 
                      TPT_WF_STACK};
   uint32_t traced = 0;
   for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
-    const uint32_t slot = qm.slot(j);
+    const uint32_t slot = queue ? W.ps.shl_keep[qm.slot(j)] : qm.slot(j);
     const float4 c = qc[slot];
-    if (!(c.w > 0.f)) continue;  // dropped (self-occluded light sample)
     const float4 o4 = qo[slot], d4 = qd[slot];
     f3 o = f3of(o4);
     float tmax = o4.w;

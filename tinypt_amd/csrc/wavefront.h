@@ -27,8 +27,9 @@ static_assert(TPT_WF_BLOCK % 64 == 0 && TPT_WF_BLOCK / 64 <= 7, "block_slot keep
 #define TPT_CNT_Q1 1
 #define TPT_CNT_SH 2   // environment shadow queue
 #define TPT_CNT_SHL 3  // area-light shadow queue
+#define TPT_CNT_SHK 4  // slots of the area-light queue entries that the light does not shadow itself
 #define TPT_CNT_IDX(q, g) (((q) * TPT_WF_SEGS + (g)) * TPT_CNT_STRIDE)
-#define TPT_CNT_ACTIVE (4 * TPT_WF_SEGS * TPT_CNT_STRIDE)    // adaptive active-pixel list (one dense queue)
+#define TPT_CNT_ACTIVE (5 * TPT_WF_SEGS * TPT_CNT_STRIDE)    // adaptive active-pixel list (one dense queue)
 #define TPT_CNT_OVERFLOW (TPT_CNT_ACTIVE + TPT_CNT_STRIDE)  // set if a segment would overflow (never by construction)
 #define TPT_NUM_CNT (TPT_CNT_OVERFLOW + TPT_CNT_STRIDE)
 #define TPT_STAT_STRIDE 32  // uint64 words of statistics per segment (256 B)
@@ -59,9 +60,10 @@ struct PathState {
   float4* sh_o[2];  // environment queue: origin, tmax ([1] unused)
   float4* sh_d[2];  // direction, path id (int bits)
   float4* sh_c[2];  // contribution if unoccluded; w: 1 (environment) or the distance to the light
-                    // sample (light; origin p + eps wi and tmax follow from it); w <= 0: dropped
+                    // sample (light; origin p + eps wi and tmax follow from it)
   float4* sh_l;     // light queue: shading point xyz, light-sample u
   float2* sh_lv;    // light queue: light-sample v, light index (int bits)
+  uint32_t* shl_keep;  // segmented queue of light-queue slots whose ray is traced (not self-occluded)
   uint32_t* counters;
   uint64_t* stats;  // per segment g: [g * TPT_STAT_STRIDE + k], k = 0 extension rays, 1 shadow rays, 2 paths
   uint32_t seg_cap; // entries per queue segment (wf_seg_cap of the allocated path count)
