@@ -492,16 +492,20 @@ struct EventTimer {
   }
 };
 
-// Bounce loop of one pass (P paths generated by the preceding raygen).
-int wf_bounces(tpt_ctx* c, const WfParams& W, uint32_t P, EventTimer& tt, uint64_t* launches) {
+// Bounce loop of one pass over the A.n_paths paths that A describes; the first bounce's kernels
+// generate the camera rays themselves.
+int wf_bounces(tpt_ctx* c, const WfParams& W, const RaygenArgs& A, EventTimer& tt, uint64_t* launches) {
   hipStream_t s = c->stream;
+  const uint32_t P = A.n_paths;
+  if (W.max_bounces == 0) HIPC(wf_launch_clear_radiance(W, P, s));
   int cur = 0;
   for (int b = 0; b < W.max_bounces; b++) {
+    const RaygenArgs* gen = b == 0 ? &A : nullptr;
     HIPC(tt.mark());
-    HIPC(wf_launch_extend(W, cur, P, s));
+    HIPC(wf_launch_extend(W, cur, gen, P, s));
     HIPC(tt.mark());
     (*launches)++;
-    HIPC(wf_launch_shade(W, cur, P, s));
+    HIPC(wf_launch_shade(W, cur, gen, P, s));
     if (W.lights.n > 0) {
       HIPC(wf_launch_light_selfocc(W, P, s));
       HIPC(wf_launch_shadow(W, 1, P, s));
@@ -553,8 +557,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       for (uint32_t s0 = 0; s0 < spp; s0 += k) {
         uint32_t kk = std::min(k, spp - s0);
         RaygenArgs A{np * kk, np, p0, s0, nullptr, nullptr, pix_map};
-        HIPC(wf_launch_raygen(W, A, s));
-        rc = wf_bounces(c, W, np * kk, tt, &launches);
+        rc = wf_bounces(c, W, A, tt, &launches);
         if (rc) return rc;
         traced += (uint64_t)np * kk;
         AccumArgs B{np, p0, kk, acc, acc_w, acc_sq};
@@ -585,8 +588,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       kk = std::min(kk, spp - taken);
       kk = std::min<uint32_t>(kk, (uint32_t)std::max<size_t>(1, P / n_active));
       RaygenArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>(), pix_map};
-      HIPC(wf_launch_raygen(W, A, s));
-      rc = wf_bounces(c, W, n_active * kk, tt, &launches);
+      rc = wf_bounces(c, W, A, tt, &launches);
       if (rc) return rc;
       traced += (uint64_t)n_active * kk;
       uint32_t* n_out = W.ps.counters + TPT_CNT_ACTIVE;

@@ -4,8 +4,9 @@
 // One pass renders `k` samples of a set of pixels: P = pixels * k paths, slot i -> pixel
 // (i mod pixels), sample (s0 + i / pixels), so neighbouring slots are neighbouring pixels.
 // Per bounce:  extend (closest hit) -> shade (miss / RR / emission / NEE sampling / BSDF
-// sampling; survivors compacted into the next queue) -> shadow (any hit for the <= 2 NEE rays
-// of each path, added to that path's radiance by the thread that owns the path).
+// sampling; survivors compacted into the next queue) -> light self-occlusion -> shadow (any hit
+// for the <= 2 NEE rays of each path, added to that path's radiance by the thread that owns the
+// path).  The first bounce's extend and shade generate the camera rays themselves.
 // After the last bounce, accumulate adds each pixel's k path radiances into FP64 sums.
 // Queue compaction: a ballot per wave and one atomic per block on the block's queue segment.
 // Random numbers: one PCG32 stream per (pixel, sample): seed_for(x, y, s) ^ seed.
@@ -60,53 +61,47 @@ __device__ __forceinline__ uint32_t seg_slot(bool pred, const PathState& ps, int
 // ---------------------------------------------------------------- kernels
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-__global__ void __launch_bounds__(TPT_WF_BLOCK) k_raygen(WfParams W, RaygenArgs A) {
-  const uint32_t P = A.n_paths;
-  for (uint32_t i = gtid(); i < P; i += gridDim.x * blockDim.x) {
-    uint32_t pix, s;
-    if (A.active) {
-      uint32_t j = i % A.n_pixels;
-      pix = (uint32_t)A.active[j];
-      s = A.nsamp[pix] + i / A.n_pixels;
-    } else {
-      pix = A.pixel0 + i % A.n_pixels;
-      s = A.s0 + i / A.n_pixels;
-    }
-    if (A.pix_map) pix = (uint32_t)A.pix_map[pix];
-    uint32_t x = pix % (uint32_t)W.width, y = pix / (uint32_t)W.width;
-    RngF32 rng(seed_for(x, y, s) ^ W.seed);
-    f3 origin, dir;
-    camera_ray32(W.cam, x, y, W.inv_w, W.inv_h, rng, &origin, &dir);
-    // chunk c of TPT_WF_BLOCK paths goes to segment c % SEGS, as the shading kernel's blocks
-    // would place it
-    const uint32_t c = i / TPT_WF_BLOCK;
-    const uint32_t slot = (c % TPT_WF_SEGS) * W.ps.seg_cap + (c / TPT_WF_SEGS) * TPT_WF_BLOCK + i % TPT_WF_BLOCK;
-    W.ps.o[0][slot] = make_float4(origin.x, origin.y, origin.z, 0.f);
-    W.ps.d[0][slot] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(0u));
-    W.ps.beta[0][slot] = make_float4(1.f, 1.f, 1.f, __int_as_float((int32_t)i));
-    W.ps.rng[0][slot] = rng.p.state;
-    W.ps.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+// Path i of a pass (render.rs:142-157): its pixel and sample, its random stream
+// seed_for(x, y, s) ^ seed, and its camera ray.  The first bounce's extension and shading
+// kernels call it for their own paths, so no path state is written for bounce 0.
+__device__ __forceinline__ void gen_path(const WfParams& W, const RaygenArgs& A, uint32_t i, RngF32* rng, f3* o, f3* d) {
+  uint32_t pix, s;
+  if (A.active) {
+    const uint32_t j = i % A.n_pixels;
+    pix = (uint32_t)A.active[j];
+    s = A.nsamp[pix] + i / A.n_pixels;
+  } else {
+    pix = A.pixel0 + i % A.n_pixels;
+    s = A.s0 + i / A.n_pixels;
   }
-  if (blockIdx.x == 0 && threadIdx.x < TPT_WF_SEGS) {
-    const uint32_t g = threadIdx.x, full = P / TPT_WF_BLOCK, rem = P % TPT_WF_BLOCK;
-    const uint32_t n_full = full > g ? (full - g + TPT_WF_SEGS - 1) / TPT_WF_SEGS : 0u;
-    W.ps.counters[TPT_CNT_IDX(TPT_CNT_Q0, g)] = n_full * TPT_WF_BLOCK + (rem && full % TPT_WF_SEGS == g ? rem : 0u);
-    W.ps.counters[TPT_CNT_IDX(TPT_CNT_Q1, g)] = 0;
-    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, g)] = 0;
-    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, g)] = 0;
-    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHK, g)] = 0;
-  }
+  if (A.pix_map) pix = (uint32_t)A.pix_map[pix];
+  const uint32_t x = pix % (uint32_t)W.width, y = pix / (uint32_t)W.width;
+  *rng = RngF32(seed_for(x, y, s) ^ W.seed);
+  camera_ray32(W.cam, x, y, W.inv_w, W.inv_h, *rng, o, d);
+}
+
+// A pass without bounces (max_bounces 0) gathers no radiance: clear its paths' sums.
+__global__ void __launch_bounds__(TPT_WF_BLOCK) k_clear_radiance(WfParams W, uint32_t n) {
+  for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) W.ps.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // OVF: the scene's stack bound exceeds the LDS part, so the traversal stack may overflow to the
-// per-thread global area (W.gstack); otherwise that path is compiled out.  Five waves per SIMD
-// fit without spills (the traversal is VALU-bound; more waves hide its load latency).
-template <bool OVF>
-__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_extend(WfParams W, int cur) {
+// per-thread global area (W.gstack); otherwise that path is compiled out.  GEN: the first
+// bounce, whose paths are the pass's camera rays, generated here (path i at position i).  Five
+// waves per SIMD fit without spills (the traversal is VALU-bound; more waves hide its load
+// latency).
+template <bool OVF, bool GEN>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+k_extend(WfParams W, int cur, RaygenArgs A) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
-  qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
-  const uint32_t n = qm.n;
+  uint32_t n;
+  if constexpr (GEN) {
+    n = A.n_paths;
+  } else {
+    qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
+    n = qm.n;
+  }
   const float4* __restrict__ po = W.ps.o[cur];
   const float4* __restrict__ pd = W.ps.d[cur];
   LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
@@ -119,10 +114,17 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
   }
   if (gtid() == 0) atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
   for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t slot = qm.slot(i);
-    float4 o4 = po[slot], d4 = pd[slot];
+    f3 o, d;
+    if constexpr (GEN) {
+      RngF32 rng(1);
+      gen_path(W, A, i, &rng, &o, &d);
+    } else {
+      const uint32_t slot = qm.slot(i);
+      o = f3of(po[slot]);
+      d = f3of(pd[slot]);
+    }
     HitF32 h;
-    trace_f32<false>(W.S, f3of(o4), f3of(d4), (float)TPT_RAY_EPSILON, 1e30f, &h, st);
+    trace_f32<false>(W.S, o, d, (float)TPT_RAY_EPSILON, 1e30f, &h, st);
     W.ps.hit[i] = make_int4(__float_as_int(h.t), h.inst, h.prim, 0);
   }
 }
@@ -133,13 +135,21 @@ __device__ __forceinline__ void add_radiance(float4* L, int32_t id, f3 c) {
 }
 
 // ENV_NEE: the environment is an image with a non-zero texel, so environment NEE runs (the
-// variant without it needs fewer registers).
-template <bool ENV_NEE>
-__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shade(WfParams W, int cur) {
+// variant without it needs fewer registers).  GEN: the first bounce; path i (position i) is
+// regenerated from its pixel and sample, as k_extend traced it, and its radiance sum L[i] is
+// written rather than added to.
+template <bool ENV_NEE, bool GEN>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_shade(WfParams W, int cur, RaygenArgs A) {
   __shared__ uint32_t s_slot[8];
   QueueMap qm;
-  qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
-  const uint32_t n = qm.n;
+  uint32_t n;
+  if constexpr (GEN) {
+    n = A.n_paths;
+  } else {
+    qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
+    n = qm.n;
+  }
   const int nxt = cur ^ 1;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -153,17 +163,29 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
     RngF32 rng(1);
     int4 h4 = make_int4(0, -1, 0, 0);
     if (i < n) {
-      const uint32_t slot = qm.slot(i);
-      float4 o4 = W.ps.o[cur][slot], d4 = W.ps.d[cur][slot], b4 = W.ps.beta[cur][slot];
+      if constexpr (GEN) {
+        gen_path(W, A, i, &rng, &o, &d);
+        beta = mk(1.f, 1.f, 1.f);
+        id = (int32_t)i;
+      } else {
+        const uint32_t slot = qm.slot(i);
+        float4 o4 = W.ps.o[cur][slot], d4 = W.ps.d[cur][slot], b4 = W.ps.beta[cur][slot];
+        o = f3of(o4);
+        d = f3of(d4);
+        beta = f3of(b4);
+        id = __float_as_int(b4.w);
+        last_pdf = o4.w;
+        flags = __float_as_uint(d4.w);
+        rng.p.state = W.ps.rng[cur][slot];
+      }
       h4 = W.ps.hit[i];
-      o = f3of(o4);
-      d = f3of(d4);
-      beta = f3of(b4);
-      id = __float_as_int(b4.w);
-      last_pdf = o4.w;
-      flags = __float_as_uint(d4.w);
-      rng.p.state = W.ps.rng[cur][slot];
     }
+    // radiance of this bounce: added to L[id], or (first bounce) gathered and written once
+    f3 l0 = mk(0.f, 0.f, 0.f);
+    auto gather = [&](f3 c) {
+      if constexpr (GEN) l0 = l0 + c;
+      else add_radiance(W.ps.L, id, c);
+    };
     const int bounce = (int)(flags & 0xffu);
     const bool last_nd = (flags & 0x100u) != 0;
     // ---- phase 1: miss, Russian roulette, emission, next-event estimation
@@ -181,7 +203,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
           f3 bg = W.env.kind == TPT_ENV_IMAGE ? env_lookup32(W.env, d) : sky32(d);
           f3 c = had(beta, bg);
           if (W.env.kind == TPT_ENV_IMAGE && last_nd) c = c * mis_weight(last_pdf, env_pdf32(W.env, d));
-          add_radiance(W.ps.L, id, clamp_luminance(c, (float)TPT_FIREFLY_CLAMP));
+          gather(clamp_luminance(c, (float)TPT_FIREFLY_CLAMP));
         }
       } else {
         go = true;
@@ -197,7 +219,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
           mp = W.mats.p32[mat];
           const f3 mc = f3of(mp);
           if (kind == TPT_MAT_DIFFUSE_LIGHT) {  // integrator.rs:100-104
-            add_radiance(W.ps.L, id, clamp_luminance(had(beta, mc), (float)TPT_FIREFLY_CLAMP));
+            gather(clamp_luminance(had(beta, mc), (float)TPT_FIREFLY_CLAMP));
             go = false;
           } else if (!mat_is_delta(kind)) {
             f3 n_o = dot(ng, d) < 0.f ? ng : -ng;  // oriented_normal (integrator.rs:138-140)
@@ -250,6 +272,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_
         }
       }
     }
+    if (GEN && i < n) W.ps.L[id] = make_float4(l0.x, l0.y, l0.z, 0.f);
     const uint32_t je = ENV_NEE ? seg_slot(has_env, W.ps, TPT_CNT_SH, s_slot) : UINT32_MAX;
     if (ENV_NEE && has_env && je != UINT32_MAX) {
       W.ps.sh_o[0][je] = sh_o0;
@@ -457,22 +480,40 @@ static uint32_t resident_blocks(const void* kernel) {
 
 static inline dim3 grid_of(uint32_t n, const void* kernel) { return dim3(std::min(grid_for(n), resident_blocks(kernel))); }
 
-hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s) {
-  hipLaunchKernelGGL(k_raygen, dim3(grid_for(A.n_paths)), dim3(TPT_WF_BLOCK), 0, s, W, A);
+hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_radiance, dim3(grid_for(n)), dim3(TPT_WF_BLOCK), 0, s, W, n);
   return hipGetLastError();
 }
-hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  if (W.gstack)
-    hipLaunchKernelGGL(k_extend<true>, grid_of(max_n, (const void*)k_extend<true>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
-  else
-    hipLaunchKernelGGL(k_extend<false>, grid_of(max_n, (const void*)k_extend<false>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+template <bool OVF, bool GEN>
+static void launch_extend(const WfParams& W, int cur, const RaygenArgs& A, uint32_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL((k_extend<OVF, GEN>), grid_of(max_n, (const void*)k_extend<OVF, GEN>), dim3(TPT_WF_BLOCK), 0, s, W,
+                     cur, A);
+}
+hipError_t wf_launch_extend(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s) {
+  const RaygenArgs A = gen ? *gen : RaygenArgs{};
+  if (W.gstack) {
+    if (gen) launch_extend<true, true>(W, cur, A, max_n, s);
+    else launch_extend<true, false>(W, cur, A, max_n, s);
+  } else {
+    if (gen) launch_extend<false, true>(W, cur, A, max_n, s);
+    else launch_extend<false, false>(W, cur, A, max_n, s);
+  }
   return hipGetLastError();
 }
-hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  if (W.env.kind == TPT_ENV_IMAGE && !W.env.black)
-    hipLaunchKernelGGL(k_shade<true>, grid_of(max_n, (const void*)k_shade<true>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
-  else
-    hipLaunchKernelGGL(k_shade<false>, grid_of(max_n, (const void*)k_shade<false>), dim3(TPT_WF_BLOCK), 0, s, W, cur);
+template <bool ENV_NEE, bool GEN>
+static void launch_shade(const WfParams& W, int cur, const RaygenArgs& A, uint32_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL((k_shade<ENV_NEE, GEN>), grid_of(max_n, (const void*)k_shade<ENV_NEE, GEN>), dim3(TPT_WF_BLOCK), 0,
+                     s, W, cur, A);
+}
+hipError_t wf_launch_shade(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s) {
+  const RaygenArgs A = gen ? *gen : RaygenArgs{};
+  if (W.env.kind == TPT_ENV_IMAGE && !W.env.black) {
+    if (gen) launch_shade<true, true>(W, cur, A, max_n, s);
+    else launch_shade<true, false>(W, cur, A, max_n, s);
+  } else {
+    if (gen) launch_shade<false, true>(W, cur, A, max_n, s);
+    else launch_shade<false, false>(W, cur, A, max_n, s);
+  }
   return hipGetLastError();
 }
 hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s) {

@@ -39,7 +39,7 @@ static_assert(TPT_WF_MAX_BLOCKS % TPT_WF_SEGS == 0, "grids are multiples of TPT_
 // Entries per queue segment for queues of at most P entries.  The shading kernel's block b
 // handles input chunks c (TPT_WF_BLOCK positions each) with c = b (mod grid), so with a grid that
 // is a multiple of TPT_WF_SEGS a segment receives the outputs of at most ceil(chunks / SEGS)
-// chunks; raygen lays the first queue out the same way.
+// chunks.
 inline uint32_t wf_seg_cap(uint32_t P) {
   const uint32_t chunks = (P + TPT_WF_BLOCK - 1) / TPT_WF_BLOCK;
   return (chunks + TPT_WF_SEGS - 1) / TPT_WF_SEGS * TPT_WF_BLOCK;
@@ -144,9 +144,10 @@ struct AdaptiveArgs {
   double* acc_sq;
 };
 
-hipError_t wf_launch_raygen(const WfParams& W, const RaygenArgs& A, hipStream_t s);
-hipError_t wf_launch_extend(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
-hipError_t wf_launch_shade(const WfParams& W, int cur, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s);
+// gen: the pass's paths, for the first bounce (its kernels generate the camera rays); null after
+hipError_t wf_launch_extend(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_shade(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s);
