@@ -51,7 +51,7 @@ def cpu_baseline(scene, seconds_target=15.0):
     16x16 tiles, Morton order, one thread per core) on a bounded sample of the workload."""
     import oracle
     o = oracle.OracleScene(scene)
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) or 1
     # calibrate on 1 spp, then size the sample to ~seconds_target
     r = o.render(W, H, 1, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED, threads=threads)
     rate = W * H / max(r["seconds"], 1e-6)
@@ -61,7 +61,19 @@ def cpu_baseline(scene, seconds_target=15.0):
     c = r["counters"]
     return {"value": paths / r["seconds"], "unit": "paths/s", "cores": threads, "kind": "port",
             "sample": f"512x512 at {spp} spp of the same scene ({paths} paths, {r['seconds']:.2f} s)",
-            "mrays_per_s": (c["rays_ext"] + c["rays_shadow"]) / r["seconds"] / 1e6}, c
+            "mrays_per_s": (c["rays_ext"] + c["rays_shadow"]) / r["seconds"] / 1e6, "cpu_model": cpu_model()}, c
+
+
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 TRAFFIC_PROFILE = "profiles/wavefront_cornell_512x512_256spp_hbm_counters.md"

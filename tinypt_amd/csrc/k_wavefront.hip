@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
 // contribution to its path's radiance.  A path has at most one ray per queue, so the
 // read-modify-write of L[id] has a single writer per launch.
 template <bool OVF>
-__global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) {
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_shadow(WfParams W, int queue) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
   qm.load(W.ps.counters, queue ? TPT_CNT_SHK : TPT_CNT_SH, W.ps.seg_cap);
@@ -350,18 +350,18 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_shadow(WfParams W, int queue) 
   uint32_t traced = 0;
   for (uint32_t j = gtid(); j < qm.n; j += gridDim.x * blockDim.x) {
     const uint32_t slot = queue ? W.ps.shl_keep[qm.slot(j)] : qm.slot(j);
-    const float4 c = qc[slot];
     const float4 o4 = qo[slot], d4 = qd[slot];
     f3 o = f3of(o4);
     float tmax = o4.w;
-    if (queue) {  // light queue: o4 is the shading point, c.w the distance to the light sample
+    if (queue) {  // light queue: o4 is the shading point, sh_c's w the distance to the light sample
       o = o + (float)TPT_RAY_EPSILON * f3of(d4);
-      tmax = light_shadow_tmax32(c.w);
+      tmax = light_shadow_tmax32(qc[slot].w);
     }
     HitF32 h;
     traced++;
+    // the contribution is read only for an unoccluded ray, after the traversal
     if (!trace_f32<true>(W.S, o, f3of(d4), (float)TPT_RAY_EPSILON, tmax, &h, st))
-      add_radiance(W.ps.L, __float_as_int(d4.w), f3of(c));
+      add_radiance(W.ps.L, __float_as_int(d4.w), f3of(qc[slot]));
   }
   // one atomic per wave for the shadow-ray count
   uint32_t tot = traced;

@@ -108,32 +108,57 @@ static_assert(offsetof(Node4, hi_x) == 16 && offsetof(Node4, lo_y) == 32 && offs
                   offsetof(Node4, lo_z) == 64 && offsetof(Node4, hi_z) == 80 && offsetof(Node4, child) == 96,
               "near_off assumes the Node4 plane layout");
 
-// Returns true on a hit.  ANY: stop at the first hit in range (shadow rays).
-// Loop shape: an inner loop descends through inner nodes (near child kept in a register, far
-// child pushed); the outer loop then handles the leaf, top-level item or instance exit that
-// the descent stopped at, and pops the next entry.
-template <bool ANY, class Stack>
-TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmin, float tmax, HitF32* hit,
-                      const Stack& st) {
-  hit->t = tmax;
-  hit->inst = -1;
-  hit->prim = -1;
-  int32_t ref = S.tlas_root;
-  if (ref == TPT_REF_EMPTY) return false;
-  const vec3<float> inv_w = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
-  const vec3<float> oi_w = mk(o.x * inv_w.x, o.y * inv_w.y, o.z * inv_w.z);
-  vec3<float> co = o, cd = d, inv = inv_w, oi = oi_w;
-  const uint32_t nx_w = near_off(inv_w.x, 0), ny_w = near_off(inv_w.y, 1), nz_w = near_off(inv_w.z, 2);
-  uint32_t nx = nx_w, ny = ny_w, nz = nz_w;
-  float lo = tmin;        // lower t bound at the current level
-  float det_eps = 1e-10f;
-  float best_t = tmax;
-  int32_t best_key = -1;
-  int32_t cur_inst = -1;  // instance being traversed (-1: top level)
-  int32_t cur_key = 0;
-  int sp = 0;
-  bool found = false;
-  for (;;) {
+// One ray's traversal, resumable between steps so that a wave can swap finished rays for new
+// ones (k_extend).  ANY: stop at the first hit in range (shadow rays).
+// A step is one pass of the loop: an inner loop descends through inner nodes (near child kept
+// in `ref`, far children pushed); the step then handles the leaf, top-level item or instance
+// exit that the descent stopped at, and pops the next entry.
+template <bool ANY>
+struct TraceF32 {
+  vec3<float> o, d, inv_w, oi_w;  // world ray
+  vec3<float> co, cd, inv, oi;    // ray of the current level (object space inside an instance)
+  uint32_t nx_w, ny_w, nz_w, nx, ny, nz;
+  float tmin;
+  float lo;       // lower t bound at the current level
+  float det_eps;
+  float best_t;
+  int32_t best_key;
+  int32_t cur_inst;  // instance being traversed (-1: top level)
+  int32_t cur_key;
+  int32_t ref;
+  int sp;
+  bool found;
+  HitF32 hit;
+
+  // False when there is nothing to traverse (the ray misses; hit is set to "no hit").
+  TPT_HD bool init(const SceneF32& S, vec3<float> ro, vec3<float> rd, float rtmin, float rtmax) {
+    hit.t = rtmax;
+    hit.inst = -1;
+    hit.prim = -1;
+    found = false;
+    ref = S.tlas_root;
+    if (ref == TPT_REF_EMPTY) return false;
+    o = ro;
+    d = rd;
+    tmin = rtmin;
+    inv_w = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    oi_w = mk(o.x * inv_w.x, o.y * inv_w.y, o.z * inv_w.z);
+    co = o; cd = d; inv = inv_w; oi = oi_w;
+    nx_w = near_off(inv_w.x, 0); ny_w = near_off(inv_w.y, 1); nz_w = near_off(inv_w.z, 2);
+    nx = nx_w; ny = ny_w; nz = nz_w;
+    lo = tmin;
+    det_eps = 1e-10f;
+    best_t = rtmax;
+    best_key = -1;
+    cur_inst = -1;
+    cur_key = 0;
+    sp = 0;
+    return true;
+  }
+
+  // One step; false when the ray is done (hit and found hold the result).
+  template <class Stack>
+  TPT_HD bool step(const SceneF32& S, const Stack& st) {
     // ---- descend inner nodes: four child boxes from one 128 B node; the nearest hit child
     // is entered, the others are pushed farthest first
     bool have = true;
@@ -162,9 +187,9 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
         const float zn = fmaf(bnz[k], inv.z, -oi.z), zf = fmaf(bfz[k], inv.z, -oi.z);
         const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, lo));
         const float tf = fminf(fminf(xf, yf), fminf(zf, best_t)) * 1.0000003f;
-        const bool hit = tn <= tf && cr[k] != TPT_REF_EMPTY;
-        dn[k] = hit ? tn : INFINITY;
-        nhit += hit ? 1 : 0;
+        const bool h = tn <= tf && cr[k] != TPT_REF_EMPTY;
+        dn[k] = h ? tn : INFINITY;
+        nhit += h ? 1 : 0;
       }
       if (nhit == 0) {
         have = false;
@@ -215,8 +240,8 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
             const int32_t key = TPT_KEY_SPHERE(s);
             if (tf < best_t || key >= best_key) {
               best_t = tf; best_key = key; found = true;
-              hit->t = tf; hit->inst = (int32_t)((uint32_t)s | TPT_ITEM_SPHERE); hit->prim = -1;
-              if (ANY) return true;
+              hit.t = tf; hit.inst = (int32_t)((uint32_t)s | TPT_ITEM_SPHERE); hit.prim = -1;
+              if (ANY) return false;
             }
           }
         } else {
@@ -227,7 +252,7 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
             cur_inst = item;
             st.push(sp, TPT_STACK_SENTINEL);
             ref = root;
-            continue;
+            return true;
           }
           if (root != TPT_REF_EMPTY) {
             const float4* w = reinterpret_cast<const float4*>(I.w2o);
@@ -247,7 +272,7 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
             cur_key = TPT_KEY_INST(item);
             st.push(sp, TPT_STACK_SENTINEL);
             ref = root;
-            continue;
+            return true;
           }
         }
       } else {
@@ -279,16 +304,29 @@ TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmi
           }
           if (th < best_t || key >= best_key) {
             best_t = th; best_key = key; found = true;
-            hit->t = th; hit->inst = hinst; hit->prim = (int32_t)(start + k);
-            if (ANY) return true;
+            hit.t = th; hit.inst = hinst; hit.prim = (int32_t)(start + k);
+            if (ANY) return false;
           }
         }
       }
     }
-    if (sp == 0) break;
+    if (sp == 0) return false;
     ref = st.pop(sp);
+    return true;
   }
-  return found;
+};
+
+// Returns true on a hit.  ANY: stop at the first hit in range (shadow rays).
+template <bool ANY, class Stack>
+TPT_HD bool trace_f32(const SceneF32& S, vec3<float> o, vec3<float> d, float tmin, float tmax, HitF32* hit,
+                      const Stack& st) {
+  TraceF32<ANY> tr;
+  if (tr.init(S, o, d, tmin, tmax)) {
+    while (tr.step(S, st)) {
+    }
+  }
+  *hit = tr.hit;
+  return tr.found;
 }
 
 }  // namespace tpt
