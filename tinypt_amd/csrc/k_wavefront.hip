@@ -87,11 +87,11 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_clear_radiance(WfParams W, uin
 
 // OVF: the scene's stack bound exceeds the LDS part, so the traversal stack may overflow to the
 // per-thread global area (W.gstack); otherwise that path is compiled out.  GEN: the first
-// bounce, whose paths are the pass's camera rays, generated here (path i at position i).  Five
-// waves per SIMD fit without spills (the traversal is VALU-bound; more waves hide its load
-// latency).
+// bounce, whose paths are the pass's camera rays, generated here (path i at position i).  Six
+// waves per SIMD fit the later bounces in 80 VGPRs without spills, five the first (its camera
+// code needs more); the traversal is VALU-bound, and more waves hide its load latency.
 template <bool OVF, bool GEN>
-__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(GEN ? 5 : 6)))
 k_extend(WfParams W, int cur, RaygenArgs A) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
