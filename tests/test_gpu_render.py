@@ -120,6 +120,27 @@ def test_black_world_renders_zero(gpu_ctx):
         assert (out.acc_w == 8).all()
 
 
+@pytest.mark.parametrize("case", ["cornell", "mesh_small"])
+@pytest.mark.parametrize("mb,rr", [(0, 3), (1, 0), (2, 1)])
+def test_short_paths_match_oracle(gpu_ctx, case, mb, rr):
+    """Paths of at most two bounces, with Russian roulette from bounce rr (integrator.rs:53-95).
+    max_bounces 0 traces nothing and gathers nothing (every pixel exactly zero, acc_w = spp).
+    With one bounce the first bounce's kernels, which generate the camera rays themselves,
+    carry the whole path: emission seen from the camera, NEE at the first hit (environment NEE
+    in the mesh scene), no scattered ray."""
+    build, w, h, *_ = PARITY_CASES[case]
+    sc = build()
+    spp = 256
+    out = T.render(T.DeviceScene(sc, gpu_ctx), T.RenderConfig(width=w, height=h, spp=spp, max_bounces=mb, rr_start=rr,
+                                                               seed=13))
+    assert (out.acc_w == spp).all()
+    if mb == 0:
+        assert not out.acc.any() and out.stats["rays_extension"] == 0
+        return
+    ref = _oracle_render(sc, w, h, spp, mb, rr, 13)
+    _check(f"{case} max_bounces {mb} rr {rr}", out, ref, out.acc_w, ref["acc_w"])
+
+
 @pytest.mark.parametrize("wh", [(1, 1), (37, 23), (3, 64)])
 def test_ragged_sizes_fp32(gpu_ctx, wh):
     w, h = wh
