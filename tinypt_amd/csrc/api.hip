@@ -408,7 +408,7 @@ int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
       HIPC(c->ps_beta[b].ensure(Q * 16));
       HIPC(c->ps_rng[b].ensure(Q * 8));
     }
-    HIPC(c->ps_hit.ensure(P * 16));
+    HIPC(c->ps_hit.ensure(P * sizeof(HitRec)));
     HIPC(c->ps_L.ensure(P * 16));
     HIPC(c->sh_o[0].ensure(Q * 16));  // the light queue keeps its origin in sh_l
     for (int q = 0; q < 2; q++) {
@@ -441,7 +441,7 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
     ps.beta[b] = c->ps_beta[b].as<float4>();
     ps.rng[b] = c->ps_rng[b].as<uint64_t>();
   }
-  ps.hit = c->ps_hit.as<int4>();
+  ps.hit = c->ps_hit.as<HitRec>();
   ps.L = c->ps_L.as<float4>();
   for (int q = 0; q < 2; q++) {
     ps.sh_o[q] = q == 0 ? c->sh_o[0].as<float4>() : nullptr;

@@ -125,7 +125,7 @@ k_extend(WfParams W, int cur, RaygenArgs A) {
     }
     HitF32 h;
     trace_f32<false>(W.S, o, d, (float)TPT_RAY_EPSILON, 1e30f, &h, st);
-    W.ps.hit[i] = make_int4(__float_as_int(h.t), h.inst, h.prim, 0);
+    W.ps.hit[i] = HitRec{__float_as_int(h.t), h.inst, h.prim};
   }
 }
 
@@ -178,7 +178,8 @@ k_shade(WfParams W, int cur, RaygenArgs A) {
         flags = __float_as_uint(d4.w);
         rng.p.state = W.ps.rng[cur][slot];
       }
-      h4 = W.ps.hit[i];
+      const HitRec hr = W.ps.hit[i];
+      h4 = make_int4(hr.t, hr.inst, hr.prim, 0);
     }
     // radiance of this bounce: added to L[id], or (first bounce) gathered and written once
     f3 l0 = mk(0.f, 0.f, 0.f);

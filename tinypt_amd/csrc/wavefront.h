@@ -45,6 +45,11 @@ inline uint32_t wf_seg_cap(uint32_t P) {
   return (chunks + TPT_WF_SEGS - 1) / TPT_WF_SEGS * TPT_WF_BLOCK;
 }
 
+// Closest hit of one extension ray: t (float bits), instance code, triangle slot (12 B).
+struct HitRec {
+  int32_t t, inst, prim;
+};
+
 // Path state, structure of arrays, in two buffer sets: the live paths of a bounce sit in the
 // segmented queue of one set; shading writes the survivors into the other set, so every kernel
 // reads its inputs coalesced.  A path carries its id (the slot of its (pixel, sample) in the
@@ -54,7 +59,7 @@ struct PathState {
   float4* d[2];     // direction xyz, flags (bits 0-7 bounce, bit 8 last bounce non-delta)
   float4* beta[2];  // throughput xyz, path id (int bits)
   uint64_t* rng[2];
-  int4* hit;        // closest hit of queue position i: t bits, instance code, triangle slot
+  HitRec* hit;      // closest hit of queue position i
   float4* L;        // radiance gathered so far, by path id
   // shadow rays, one segmented queue per kind: [0] environment, [1] area light
   float4* sh_o[2];  // environment queue: origin, tmax ([1] unused)
