@@ -494,13 +494,13 @@ struct EventTimer {
 
 // Bounce loop of one pass over the A.n_paths paths that A describes; the first bounce's kernels
 // generate the camera rays themselves.
-int wf_bounces(tpt_ctx* c, const WfParams& W, const RaygenArgs& A, EventTimer& tt, uint64_t* launches) {
+int wf_bounces(tpt_ctx* c, const WfParams& W, const PassArgs& A, EventTimer& tt, uint64_t* launches) {
   hipStream_t s = c->stream;
   const uint32_t P = A.n_paths;
   if (W.max_bounces == 0) HIPC(wf_launch_clear_radiance(W, P, s));
   int cur = 0;
   for (int b = 0; b < W.max_bounces; b++) {
-    const RaygenArgs* gen = b == 0 ? &A : nullptr;
+    const PassArgs* gen = b == 0 ? &A : nullptr;
     HIPC(tt.mark());
     HIPC(wf_launch_extend(W, cur, gen, P, s));
     HIPC(tt.mark());
@@ -556,7 +556,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       uint32_t np = std::min(pix_chunk, npix - p0);
       for (uint32_t s0 = 0; s0 < spp; s0 += k) {
         uint32_t kk = std::min(k, spp - s0);
-        RaygenArgs A{np * kk, np, p0, s0, nullptr, nullptr, pix_map};
+        PassArgs A{np * kk, np, p0, s0, nullptr, nullptr, pix_map};
         rc = wf_bounces(c, W, A, tt, &launches);
         if (rc) return rc;
         traced += (uint64_t)np * kk;
@@ -587,7 +587,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       kk = std::max(kk, std::min<uint32_t>(fill, 64));
       kk = std::min(kk, spp - taken);
       kk = std::min<uint32_t>(kk, (uint32_t)std::max<size_t>(1, P / n_active));
-      RaygenArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>(), pix_map};
+      PassArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>(), pix_map};
       rc = wf_bounces(c, W, A, tt, &launches);
       if (rc) return rc;
       traced += (uint64_t)n_active * kk;

@@ -64,7 +64,7 @@ __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + th
 // Path i of a pass (render.rs:142-157): its pixel and sample, its random stream
 // seed_for(x, y, s) ^ seed, and its camera ray.  The first bounce's extension and shading
 // kernels call it for their own paths, so no path state is written for bounce 0.
-__device__ __forceinline__ void gen_path(const WfParams& W, const RaygenArgs& A, uint32_t i, RngF32* rng, f3* o, f3* d) {
+__device__ __forceinline__ void gen_path(const WfParams& W, const PassArgs& A, uint32_t i, RngF32* rng, f3* o, f3* d) {
   uint32_t pix, s;
   if (A.active) {
     const uint32_t j = i % A.n_pixels;
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_clear_radiance(WfParams W, uin
 // code needs more); the traversal is VALU-bound, and more waves hide its load latency.
 template <bool OVF, bool GEN>
 __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(GEN ? 5 : 6)))
-k_extend(WfParams W, int cur, RaygenArgs A) {
+k_extend(WfParams W, int cur, PassArgs A) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
   uint32_t n;
@@ -140,7 +140,7 @@ __device__ __forceinline__ void add_radiance(float4* L, int32_t id, f3 c) {
 // written rather than added to.
 template <bool ENV_NEE, bool GEN>
 __global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-k_shade(WfParams W, int cur, RaygenArgs A) {
+k_shade(WfParams W, int cur, PassArgs A) {
   __shared__ uint32_t s_slot[8];
   QueueMap qm;
   uint32_t n;
@@ -486,12 +486,12 @@ hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s
   return hipGetLastError();
 }
 template <bool OVF, bool GEN>
-static void launch_extend(const WfParams& W, int cur, const RaygenArgs& A, uint32_t max_n, hipStream_t s) {
+static void launch_extend(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s) {
   hipLaunchKernelGGL((k_extend<OVF, GEN>), grid_of(max_n, (const void*)k_extend<OVF, GEN>), dim3(TPT_WF_BLOCK), 0, s, W,
                      cur, A);
 }
-hipError_t wf_launch_extend(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s) {
-  const RaygenArgs A = gen ? *gen : RaygenArgs{};
+hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s) {
+  const PassArgs A = gen ? *gen : PassArgs{};
   if (W.gstack) {
     if (gen) launch_extend<true, true>(W, cur, A, max_n, s);
     else launch_extend<true, false>(W, cur, A, max_n, s);
@@ -502,12 +502,12 @@ hipError_t wf_launch_extend(const WfParams& W, int cur, const RaygenArgs* gen, u
   return hipGetLastError();
 }
 template <bool ENV_NEE, bool GEN>
-static void launch_shade(const WfParams& W, int cur, const RaygenArgs& A, uint32_t max_n, hipStream_t s) {
+static void launch_shade(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s) {
   hipLaunchKernelGGL((k_shade<ENV_NEE, GEN>), grid_of(max_n, (const void*)k_shade<ENV_NEE, GEN>), dim3(TPT_WF_BLOCK), 0,
                      s, W, cur, A);
 }
-hipError_t wf_launch_shade(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s) {
-  const RaygenArgs A = gen ? *gen : RaygenArgs{};
+hipError_t wf_launch_shade(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s) {
+  const PassArgs A = gen ? *gen : PassArgs{};
   if (W.env.kind == TPT_ENV_IMAGE && !W.env.black) {
     if (gen) launch_shade<true, true>(W, cur, A, max_n, s);
     else launch_shade<true, false>(W, cur, A, max_n, s);

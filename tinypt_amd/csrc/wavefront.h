@@ -119,7 +119,9 @@ struct WfParams {
   uint64_t seed;
 };
 
-struct RaygenArgs {
+// The paths of one pass: path i is pixel (i mod n_pixels) of the pass's pixel set, sample
+// (i / n_pixels) of the samples the pass adds to it.
+struct PassArgs {
   uint32_t n_paths;   // pixels * k
   uint32_t n_pixels;  // pixels in this pass
   uint32_t pixel0;    // first pixel (non-adaptive)
@@ -151,8 +153,8 @@ struct AdaptiveArgs {
 
 hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s);
 // gen: the pass's paths, for the first bounce (its kernels generate the camera rays); null after
-hipError_t wf_launch_extend(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s);
-hipError_t wf_launch_shade(const WfParams& W, int cur, const RaygenArgs* gen, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
+hipError_t wf_launch_shade(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s);
