@@ -265,6 +265,21 @@ RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf) {
   return std::move(b.out);
 }
 
+// Boxes of the frontier of a tree at depth `max_depth` (with the leaves above it): together they
+// hold every primitive.  Through an instance's rotation their corners bound the instance much
+// more tightly than the root box's eight corners, whose world box grows by up to sqrt(3) per
+// axis for a rotated round mesh -- rays would then enter instances they cannot hit.
+static constexpr int kCutDepth = 6;  // up to 64 boxes per mesh
+static void tree_cut(const RefBvh& t, int32_t node, int depth, int max_depth, std::vector<DAabb>* out) {
+  const RefNode& n = t.nodes[(size_t)node];
+  if (depth >= max_depth || n.left < 0) {
+    out->push_back(n.bbox);
+    return;
+  }
+  tree_cut(t, n.left, depth + 1, max_depth, out);
+  tree_cut(t, n.right, depth + 1, max_depth, out);
+}
+
 // ------------------------------------------------------------------ FP32 conversion
 static inline float round_down(double x) {
   float f = (float)x;
@@ -535,6 +550,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
 
   // ---- per-mesh BLAS: reference build and layout (plain FP64 form); own SAH tree, reordered
   // FP32 triangles and Node2 (FP32 forms)
+  std::vector<std::vector<DAabb>> mesh_cut((size_t)nmesh);  // bounding boxes for the instances' world boxes
   blas.resize((size_t)nmesh);
   meshes.resize((size_t)nmesh);
   mesh_root2.assign((size_t)nmesh, TPT_REF_EMPTY);
@@ -580,6 +596,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
     if (!f.nodes.empty()) {
       mesh_root2[(size_t)m] = to_node2(f, 0, b0, nodes2, error);
       if (!error.empty()) return false;
+      tree_cut(f, 0, 0, kCutDepth, &mesh_cut[(size_t)m]);
     }
     meshes[(size_t)m] = md;
   }
@@ -617,15 +634,16 @@ bool HostScene::build(const tpt_scene_desc* d) {
     y.mesh = valid ? x.mesh : -1;
     y.pad = 0;
     if (!valid || blas[(size_t)x.mesh].nodes.empty()) continue;
-    // world box: the eight corners of the mesh root box through A, t; padded
-    const DAabb& ob = blas[(size_t)x.mesh].nodes[0].bbox;
+    // world box: the corners of the mesh's cut boxes through A, t; padded
     DAabb wb = empty_box();
-    for (int c = 0; c < 8; c++) {
-      double p[3] = {(c & 1) ? ob.mx[0] : ob.mn[0], (c & 2) ? ob.mx[1] : ob.mn[1], (c & 4) ? ob.mx[2] : ob.mn[2]};
-      double q[3];
-      mat_vec(x.a, p, q);
-      for (int k = 0; k < 3; k++) q[k] += x.t[k];
-      wb = grow(wb, q);
+    for (const DAabb& ob : mesh_cut[(size_t)x.mesh]) {
+      for (int c = 0; c < 8; c++) {
+        double p[3] = {(c & 1) ? ob.mx[0] : ob.mn[0], (c & 2) ? ob.mx[1] : ob.mn[1], (c & 4) ? ob.mx[2] : ob.mn[2]};
+        double q[3];
+        mat_vec(x.a, p, q);
+        for (int k = 0; k < 3; k++) q[k] += x.t[k];
+        wb = grow(wb, q);
+      }
     }
     double ext = std::max({wb.mx[0] - wb.mn[0], wb.mx[1] - wb.mn[1], wb.mx[2] - wb.mn[2], 1e-6});
     for (int k = 0; k < 3; k++) {
