@@ -63,3 +63,20 @@ extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const flo
   *stack_max = max_sp;
   return 0;
 }
+
+// World box of an instance of a triangle mesh as the scene build makes it (the cut of the mesh's
+// FP32 tree through A, t), and, for comparison, the box of the root box's eight corners.
+extern "C" int tpt_test_instance_box(int64_t n_tris, const double* tri_v, const double* a, const double* t,
+                                     double* box_cut, double* box_root) {
+  std::vector<DAabb> bounds((size_t)n_tris);
+  for (int64_t i = 0; i < n_tris; i++) bounds[(size_t)i] = triangle_bounds(tri_v + 9 * i);
+  const RefBvh tree = build_sah_bvh(bounds, 4);
+  if (tree.nodes.empty()) return -1;
+  const DAabb c = instance_world_box(tree_cut_boxes(tree, kInstanceCutDepth), a, t);
+  const DAabb r = instance_world_box(std::vector<DAabb>{tree.nodes[0].bbox}, a, t);
+  for (int k = 0; k < 3; k++) {
+    box_cut[k] = c.mn[k]; box_cut[3 + k] = c.mx[k];
+    box_root[k] = r.mn[k]; box_root[3 + k] = r.mx[k];
+  }
+  return 0;
+}

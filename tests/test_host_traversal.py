@@ -111,3 +111,31 @@ def test_flattening_keeps_mirrored_instance_normals():
     b.mesh_instance(scenes.unit_cube_tris(), scenes.compose_ops([Transform.translate((2.0, 0, 0))]), Lambert((0.5,) * 3))
     sc = b.build(Camera.look_at((0, 0, 5), (0, 0, 0), (0, 1, 0), 40, 1), EnvMap.constant((1.0, 1.0, 1.0)))
     _check(sc, _rays(rng, 20_000, (-3, -3, -3), (3, 3, 3), np.array([[0.0, 0, 0], [2.0, 0, 0]])), True)
+
+
+def test_instance_world_boxes_are_tight_and_conservative():
+    """An instance's top-level box is built from the corners of a cut of its mesh tree through the
+    instance transform: it contains every transformed vertex (the traversal may skip the instance
+    otherwise) and, for rotated round meshes, stays close to the exact box, where the root box's
+    eight corners grow it by up to sqrt(3) per axis."""
+    rng = np.random.default_rng(6)
+    tris, _ = scenes.displaced_icosphere(4, 0.05, 7)
+    tv = np.ascontiguousarray(tris.reshape(-1, 9), dtype=np.float64)
+    lib = _lib()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    worst_cut, mean_root, mean_cut = 0.0, [], []
+    for _ in range(40):
+        a = np.ascontiguousarray(scenes.random_rotation(rng) * rng.uniform(0.2, 1.0))
+        t = rng.uniform(-25, 25, 3)
+        bc, br = np.zeros(6), np.zeros(6)
+        assert lib.tpt_test_instance_box(C.c_int64(len(tv)), dp(tv), dp(a), dp(t), dp(bc), dp(br)) == 0
+        w = tris.reshape(-1, 3) @ a.T + t
+        lo, hi = w.min(0), w.max(0)
+        assert np.all(bc[:3] <= lo) and np.all(bc[3:] >= hi)  # conservative
+        ext = hi - lo
+        worst_cut = max(worst_cut, float(((bc[3:] - bc[:3]) / ext).max()))
+        mean_cut.append(float(((bc[3:] - bc[:3]) / ext).mean()))
+        mean_root.append(float(((br[3:] - br[:3]) / ext).mean()))
+    print(f"cut box / exact extent: worst {worst_cut:.3f}, mean {np.mean(mean_cut):.3f}; root-corner box / exact: mean {np.mean(mean_root):.3f}")
+    assert worst_cut < 1.15 and np.mean(mean_cut) < 1.1
+    assert np.mean(mean_root) > 1.3

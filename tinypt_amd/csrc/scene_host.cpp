@@ -265,21 +265,6 @@ RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf) {
   return std::move(b.out);
 }
 
-// Boxes of the frontier of a tree at depth `max_depth` (with the leaves above it): together they
-// hold every primitive.  Through an instance's rotation their corners bound the instance much
-// more tightly than the root box's eight corners, whose world box grows by up to sqrt(3) per
-// axis for a rotated round mesh -- rays would then enter instances they cannot hit.
-static constexpr int kCutDepth = 6;  // up to 64 boxes per mesh
-static void tree_cut(const RefBvh& t, int32_t node, int depth, int max_depth, std::vector<DAabb>* out) {
-  const RefNode& n = t.nodes[(size_t)node];
-  if (depth >= max_depth || n.left < 0) {
-    out->push_back(n.bbox);
-    return;
-  }
-  tree_cut(t, n.left, depth + 1, max_depth, out);
-  tree_cut(t, n.right, depth + 1, max_depth, out);
-}
-
 // ------------------------------------------------------------------ FP32 conversion
 static inline float round_down(double x) {
   float f = (float)x;
@@ -470,6 +455,42 @@ static void mat_vec(const double* m, const double* v, double* out) {
   out[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
 }
 
+static void tree_cut(const RefBvh& t, int32_t node, int depth, int max_depth, std::vector<DAabb>* out) {
+  const RefNode& n = t.nodes[(size_t)node];
+  if (depth >= max_depth || n.left < 0) {
+    out->push_back(n.bbox);
+    return;
+  }
+  tree_cut(t, n.left, depth + 1, max_depth, out);
+  tree_cut(t, n.right, depth + 1, max_depth, out);
+}
+
+std::vector<DAabb> tree_cut_boxes(const RefBvh& t, int max_depth) {
+  std::vector<DAabb> out;
+  if (!t.nodes.empty()) tree_cut(t, 0, 0, max_depth, &out);
+  return out;
+}
+
+DAabb instance_world_box(const std::vector<DAabb>& cut, const double* a, const double* t) {
+  DAabb wb = empty_box();
+  for (const DAabb& ob : cut) {
+    for (int c = 0; c < 8; c++) {
+      double p[3] = {(c & 1) ? ob.mx[0] : ob.mn[0], (c & 2) ? ob.mx[1] : ob.mn[1], (c & 4) ? ob.mx[2] : ob.mn[2]};
+      double q[3];
+      mat_vec(a, p, q);
+      for (int k = 0; k < 3; k++) q[k] += t[k];
+      wb = grow(wb, q);
+    }
+  }
+  const double ext = std::max({wb.mx[0] - wb.mn[0], wb.mx[1] - wb.mn[1], wb.mx[2] - wb.mn[2], 1e-6});
+  for (int k = 0; k < 3; k++) {
+    const double pad = 1e-6 * ext + 1e-6 * std::max(std::fabs(wb.mn[k]), std::fabs(wb.mx[k]));
+    wb.mn[k] -= pad;
+    wb.mx[k] += pad;
+  }
+  return wb;
+}
+
 static double mat_det(const double* m) {
   return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
@@ -596,7 +617,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
     if (!f.nodes.empty()) {
       mesh_root2[(size_t)m] = to_node2(f, 0, b0, nodes2, error);
       if (!error.empty()) return false;
-      tree_cut(f, 0, 0, kCutDepth, &mesh_cut[(size_t)m]);
+      mesh_cut[(size_t)m] = tree_cut_boxes(f, kInstanceCutDepth);
     }
     meshes[(size_t)m] = md;
   }
@@ -634,23 +655,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
     y.mesh = valid ? x.mesh : -1;
     y.pad = 0;
     if (!valid || blas[(size_t)x.mesh].nodes.empty()) continue;
-    // world box: the corners of the mesh's cut boxes through A, t; padded
-    DAabb wb = empty_box();
-    for (const DAabb& ob : mesh_cut[(size_t)x.mesh]) {
-      for (int c = 0; c < 8; c++) {
-        double p[3] = {(c & 1) ? ob.mx[0] : ob.mn[0], (c & 2) ? ob.mx[1] : ob.mn[1], (c & 4) ? ob.mx[2] : ob.mn[2]};
-        double q[3];
-        mat_vec(x.a, p, q);
-        for (int k = 0; k < 3; k++) q[k] += x.t[k];
-        wb = grow(wb, q);
-      }
-    }
-    double ext = std::max({wb.mx[0] - wb.mn[0], wb.mx[1] - wb.mn[1], wb.mx[2] - wb.mn[2], 1e-6});
-    for (int k = 0; k < 3; k++) {
-      double pad = 1e-6 * ext + 1e-6 * std::max(std::fabs(wb.mn[k]), std::fabs(wb.mx[k]));
-      wb.mn[k] -= pad;
-      wb.mx[k] += pad;
-    }
+    const DAabb wb = instance_world_box(mesh_cut[(size_t)x.mesh], x.a, x.t);
     item_box.push_back(wb);
     item_code.push_back(i);
   }

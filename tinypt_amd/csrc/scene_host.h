@@ -48,6 +48,17 @@ RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf);
 // Triangle::bounds (geometry.rs:221-225) of a static triangle.
 DAabb triangle_bounds(const double* v9);
 
+// Boxes of the frontier of a tree at depth `max_depth` (with the leaves above it): together they
+// hold every primitive.  Through an instance's rotation their corners bound the instance much
+// more tightly than the root box's eight corners, whose world box grows by up to sqrt(3) per
+// axis for a rotated round mesh -- rays would then enter instances they cannot hit.
+constexpr int kInstanceCutDepth = 10;  // up to 1024 boxes per mesh
+std::vector<DAabb> tree_cut_boxes(const RefBvh& t, int max_depth);
+
+// World box of an instance (A, t) of a mesh with cut boxes `cut`: the box of their corners
+// through the transform, padded by 1e-6 relative.
+DAabb instance_world_box(const std::vector<DAabb>& cut, const double* a, const double* t);
+
 // Scenes with at most this many instanced triangles are flattened into world space.
 constexpr int64_t kFlatMaxTris = 1 << 20;
 
