@@ -52,15 +52,18 @@ def _params(sc):
     return out + [(i, 3) for i, m in enumerate(sc.mats) if m.kind == _abi.MAT_GGX]
 
 
-@pytest.mark.parametrize("name,rr", [("material_zoo", 6), ("cornell", 6), ("material_zoo_rr", 2)])
-def test_adjoint_matches_finite_differences(gpu_ctx, name, rr):
+# Paths of up to 9 bounces are summed and differentiated in one replay (their BSDF vertices are
+# logged until the path total is known); deeper ones in two ("material_zoo_deep").
+@pytest.mark.parametrize("name,rr,mb", [("material_zoo", 6, 6), ("cornell", 6, 6), ("material_zoo_rr", 2, 6),
+                                        ("material_zoo_deep", 12, 12)])
+def test_adjoint_matches_finite_differences(gpu_ctx, name, rr, mb):
     if name == "cornell":
         sc, w, h = scenes.cornell_box(32, 32), 32, 32
     else:
         sc, w, h = material_zoo(), 32, 32
     ds, base, dl = _setup(gpu_ctx, sc, w, h)
-    cfg = T.RenderConfig(width=w, height=h, spp=64, max_bounces=6, rr_start=rr, seed=21)
-    if rr < 6:  # with Russian roulette only roughness is compared (its estimator is FD-consistent)
+    cfg = T.RenderConfig(width=w, height=h, spp=64, max_bounces=mb, rr_start=rr, seed=21)
+    if rr < mb:  # with Russian roulette only roughness is compared (its estimator is FD-consistent)
         pairs = [(i, 3) for i, m in enumerate(sc.mats) if m.kind == _abi.MAT_GGX]
     else:
         pairs = _params(sc)

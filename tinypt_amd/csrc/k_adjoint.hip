@@ -42,10 +42,22 @@ namespace tpt {
 
 // Replay modes: SUM returns S_total (and lists the path's GGX materials), GRAD accumulates the
 // gradients, VALUE returns the path's loss contribution delta . L with one material's roughness
-// shifted.
+// shifted.  SUMGRAD does SUM and GRAD in one replay: a BSDF vertex's suffix term needs S_total,
+// so the vertex's factors and S_upto are logged and the terms added once the path is done --
+// possible when a path has at most TPT_ADJ_LOG BSDF vertices (max_bounces <= TPT_ADJ_LOG + 1).
 #define ADJ_SUM 0
 #define ADJ_GRAD 1
 #define ADJ_VALUE 2
+#define ADJ_SUMGRAD 3
+#define TPT_ADJ_LOG 8
+
+// BSDF vertices of one path whose weight gradient waits for S_total: the term for channel c is
+// (S_total - S_upto)_c * num_c / den_c, skipped where den_c = 0.
+struct VertexLog {
+  int n;
+  int mat[TPT_ADJ_LOG];
+  f3 num[TPT_ADJ_LOG], den[TPT_ADJ_LOG], s_upto[TPT_ADJ_LOG];
+};
 
 struct GgxList {
   int n;
@@ -113,15 +125,21 @@ struct GradSink {
   }
 };
 
+__device__ __forceinline__ void add_suffix_terms(GradSink& sink, int mat, f3 num, f3 den, f3 suf) {
+  if (den.x != 0.f) sink.add(mat, 0, suf.x * num.x / den.x);
+  if (den.y != 0.f) sink.add(mat, 1, suf.y * num.y / den.y);
+  if (den.z != 0.f) sink.add(mat, 2, suf.z * num.z / den.z);
+}
+
 // One path (see the modes above).  VALUE: material pert_mat's roughness is replaced by
 // pert_alpha (the FP32 rounding of the FP64 value +- h, as a material update would upload it).
 // The mode is a run-time value, so the kernel holds one inlined copy of the replay: a copy per
 // mode took 256 VGPRs plus 620 B of private memory per lane and ran one wave per SIMD; one copy
 // runs three waves per SIMD (168 VGPRs, some spilled) and the kernel 2.3x faster.
 __device__ __forceinline__ f3 replay(const int MODE, const AdjParams& A, uint32_t x, uint32_t y, uint32_t s, f3 delta,
-                                     f3 S_total, GradSink& sink, GgxList* ggx, int pert_mat, float pert_alpha,
-                                     const LaneStack& st, uint32_t* n_ext, uint32_t* n_sh) {
-  const bool GRAD = MODE == ADJ_GRAD;
+                                     f3 S_total, GradSink& sink, GgxList* ggx, VertexLog* log, int pert_mat,
+                                     float pert_alpha, const LaneStack& st, uint32_t* n_ext, uint32_t* n_sh) {
+  const bool GRAD = MODE == ADJ_GRAD || MODE == ADJ_SUMGRAD;
   RngF32 rng(seed_for(x, y, s) ^ A.seed);
   f3 o, d;
   camera_ray32(A.cam, x, y, A.inv_w, A.inv_h, rng, &o, &d);
@@ -170,7 +188,7 @@ __device__ __forceinline__ f3 replay(const int MODE, const AdjParams& A, uint32_
       }
       break;
     }
-    if (MODE == ADJ_SUM && kind == TPT_MAT_GGX) ggx->add(mat);
+    if (ggx && kind == TPT_MAT_GGX) ggx->add(mat);
     f3 n = dot(ng, d) < 0.f ? ng : -ng;
     f3 wo = vnorm(-d);
     if (!mat_is_delta(kind)) {
@@ -255,27 +273,36 @@ __device__ __forceinline__ f3 replay(const int MODE, const AdjParams& A, uint32_
     BsdfSample<float> bs;
     if (!bsdf_sample(kind, mc, mp.w, d, p, ng, h.t, rng, &bs)) break;
     if (GRAD) {
-      // the sampled weight scales every later term: d log w / d theta times the suffix sum
-      f3 suf = S_total - S_upto;
+      // the sampled weight scales every later term: d log w / d theta times the suffix sum,
+      // (S_total - S_upto)_c * num_c / den_c
+      f3 num = mk(0.f, 0.f, 0.f), den = num;
       if (kind == TPT_MAT_LAMBERT || kind == TPT_MAT_METAL || kind == TPT_MAT_SUBSURFACE) {
-        if (mc.x > 0.f) sink.add(mat, 0, suf.x / mc.x);
-        if (mc.y > 0.f) sink.add(mat, 1, suf.y / mc.y);
-        if (mc.z > 0.f) sink.add(mat, 2, suf.z / mc.z);
+        num = mk(1.f, 1.f, 1.f);  // d log w_c / d albedo_c = 1 / albedo_c
+        den = mk(mc.x > 0.f ? mc.x : 0.f, mc.y > 0.f ? mc.y : 0.f, mc.z > 0.f ? mc.z : 0.f);
       } else if (kind == TPT_MAT_GGX) {
         // w_c = F_c D G co / (denom pdf): d log w_c / dF0_c = (1 - (1 - |wi.m|)^5) / F_c
         f3 m = vnorm(wo + bs.d);
         float c = fabsf(dot(bs.d, m));
         f3 F = fresnel_schlick(c, mc);
         float dF = 1.f - pow5(1.f - fmaxf(c, 0.f));
-        if (F.x > 0.f) sink.add(mat, 0, suf.x * dF / F.x);
-        if (F.y > 0.f) sink.add(mat, 1, suf.y * dF / F.y);
-        if (F.z > 0.f) sink.add(mat, 2, suf.z * dF / F.z);
+        num = mk(dF, dF, dF);
+        den = mk(F.x > 0.f ? F.x : 0.f, F.y > 0.f ? F.y : 0.f, F.z > 0.f ? F.z : 0.f);
       } else if (kind == TPT_MAT_DIELECTRIC && dot(ng, d) >= 0.f) {
         // exiting: weight carries exp(-sigma_c t) (material.rs:111-119)
         float dist = fmaxf(h.t, 0.f);
-        sink.add(mat, 0, -dist * suf.x);
-        sink.add(mat, 1, -dist * suf.y);
-        sink.add(mat, 2, -dist * suf.z);
+        num = mk(-dist, -dist, -dist);
+        den = mk(1.f, 1.f, 1.f);
+      }
+      if (den.x != 0.f || den.y != 0.f || den.z != 0.f) {
+        if (log) {
+          const int j = log->n++;  // fits: a SUMGRAD path has at most TPT_ADJ_LOG BSDF vertices
+          log->mat[j] = mat;
+          log->num[j] = num;
+          log->den[j] = den;
+          log->s_upto[j] = S_upto;
+        } else {
+          add_suffix_terms(sink, mat, num, den, S_total - S_upto);
+        }
       }
     }
     T = had(T, bs.weight);
@@ -297,6 +324,7 @@ __global__ void __launch_bounds__(TPT_ADJ_BLOCK) __attribute__((amdgpu_waves_per
                TPT_WF_STACK_ADJ};
   GradSink sink{s_grad, A.grad, A.n_mats};
   const uint32_t npix = (uint32_t)A.width * (uint32_t)A.height;
+  const bool one_pass = A.max_bounces <= TPT_ADJ_LOG + 1;  // at most max_bounces - 1 BSDF vertices
   uint32_t n_ext = 0, n_sh = 0, n_paths = 0;
   for (uint64_t i = gid; i < A.n_paths; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t pix = (uint32_t)(i % npix), s = (uint32_t)(i / npix);
@@ -305,21 +333,28 @@ __global__ void __launch_bounds__(TPT_ADJ_BLOCK) __attribute__((amdgpu_waves_per
     if (delta.x == 0.f && delta.y == 0.f && delta.z == 0.f) continue;
     GgxList ggx;
     ggx.n = 0;
+    VertexLog log;
+    log.n = 0;
     const f3 zero = mk(0.f, 0.f, 0.f);
-    // pass 0 sums the path's terms (and lists its GGX materials), pass 1 takes the gradients,
-    // then two passes per GGX material replay the path with its roughness at +h and -h
+    // pass 0 sums the path's terms (and lists its GGX materials) -- and, when the path is short
+    // enough to log its BSDF vertices, takes the gradients too; otherwise pass 1 takes them.
+    // Then two passes per GGX material replay the path with its roughness at +h and -h.
+    const int first_value = one_pass ? 1 : 2;
     f3 S_total = zero, dv = zero;
-    for (int pass = 0; pass < 2 + 2 * ggx.n; pass++) {
-      const int mode = pass == 0 ? ADJ_SUM : pass == 1 ? ADJ_GRAD : ADJ_VALUE;
-      const int m = pass >= 2 ? ggx.mat[(pass - 2) >> 1] : -1;
-      const float alpha = pass >= 2 ? A.alpha_pm[2 * m + (pass & 1)] : 0.f;
-      const f3 v = replay(mode, A, x, y, s, delta, pass == 1 ? S_total : zero, sink, pass == 0 ? &ggx : nullptr, m,
-                          alpha, st, &n_ext, &n_sh);
+    for (int pass = 0; pass < first_value + 2 * ggx.n; pass++) {
+      const int mode = pass == 0 ? (one_pass ? ADJ_SUMGRAD : ADJ_SUM) : pass < first_value ? ADJ_GRAD : ADJ_VALUE;
+      const int vi = pass - first_value;
+      const int m = vi >= 0 ? ggx.mat[vi >> 1] : -1;
+      const float alpha = vi >= 0 ? A.alpha_pm[2 * m + (vi & 1)] : 0.f;
+      const f3 v = replay(mode, A, x, y, s, delta, pass == 1 && !one_pass ? S_total : zero, sink,
+                          pass == 0 ? &ggx : nullptr, pass == 0 && one_pass ? &log : nullptr, m, alpha, st, &n_ext,
+                          &n_sh);
       if (pass == 0) {
         S_total = v;
-      } else if (pass >= 2) {  // loss term at alpha + h minus the one at alpha - h
-        dv = (pass & 1) ? dv - v : v;
-        if (pass & 1) sink.add(m, 3, (dv.x + dv.y + dv.z) * (float)(0.5 / TPT_ADJ_ALPHA_H));
+        for (int j = 0; j < log.n; j++) add_suffix_terms(sink, log.mat[j], log.num[j], log.den[j], S_total - log.s_upto[j]);
+      } else if (vi >= 0) {  // loss term at alpha + h minus the one at alpha - h
+        dv = (vi & 1) ? dv - v : v;
+        if (vi & 1) sink.add(m, 3, (dv.x + dv.y + dv.z) * (float)(0.5 / TPT_ADJ_ALPHA_H));
       }
     }
     n_paths++;
