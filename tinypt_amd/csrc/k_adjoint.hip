@@ -3,7 +3,8 @@
 //
 // Every (pixel, sample) path is regenerated from its seed, seed_for(x, y, s) ^ seed, with the
 // same random-number consumption as the forward wavefront kernels, so the adjoint sees exactly
-// the paths the forward render summed.  Nothing is stored per path: each path is walked twice.
+// the paths the forward render summed.  Nothing is stored between paths; passes A and B below
+// are one walk for paths short enough to log their BSDF vertices (ADJ_SUMGRAD), two otherwise.
 //   Pass A sums, over the path's radiance terms c_k (misses, emitter hits, unoccluded NEE),
 //     s_k = x_k o (delta^T J_k), where x_k is the term before the firefly clamp, J_k the clamp's
 //     Jacobian (identity when not clamped) and delta = dLoss/dPixel / spp.
