@@ -78,6 +78,10 @@ struct tpt_ctx {
       gstack;
   // adaptive / plain per-pixel state
   DevBuf act0, act1, nsamp, welford, rngpix, done;
+  // adjoint (grow-only, kept between calls)
+  DevBuf adj[ADJ_SCRATCH_N];
+  // device sums of renders into host buffers (grow-only)
+  DevBuf out_acc, out_w, out_sq;
   size_t ps_cap = 0;
 };
 
@@ -708,7 +712,7 @@ int render_device(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg,
 // Host-side sums for a domain of npix pixels: device sums, render, copy back.
 int render_to_host(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, uint32_t npix, const int32_t* pix_map,
                    double* acc_rgb, double* acc_w, double* acc_sq, tpt_stats* st) {
-  DevBuf a, w, q;
+  DevBuf &a = c->out_acc, &w = c->out_w, &q = c->out_sq;
   HIPC(a.ensure((size_t)npix * 24));
   HIPC(w.ensure((size_t)npix * 8));
   if (acc_sq) HIPC(q.ensure((size_t)npix * 24));
@@ -800,8 +804,21 @@ int tpt_render_adjoint(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_co
   int rc = check_cfg(cfg);
   if (rc) return rc;
   HIPC(hipSetDevice(ctx->device));
+  size_t bytes[ADJ_SCRATCH_N];
+  adjoint_scratch_bytes(cfg, scene->mats.n, stack_need(scene->host), bytes);
+  void* p[ADJ_SCRATCH_N];
+  for (int k = 0; k < ADJ_SCRATCH_N; k++) {
+    p[k] = nullptr;
+    if (bytes[k] > 0) {
+      HIPC(ctx->adj[k].ensure(bytes[k]));
+      p[k] = ctx->adj[k].p;
+    }
+  }
+  const AdjScratch scratch{p[ADJ_SCRATCH_DLDP], p[ADJ_SCRATCH_GRAD], p[ADJ_SCRATCH_STATS], p[ADJ_SCRATCH_APM],
+                           p[ADJ_SCRATCH_GSTACK]};
   return adjoint_render(ctx->stream, scene->s32, scene->s64, scene->env, scene->lights, scene->mats, scene->cam32,
-                        scene->host.mat_param.data(), stack_need(scene->host), cfg, d_loss_d_pixel, grad_params, stats, &g_err);
+                        scene->host.mat_param.data(), stack_need(scene->host), cfg, d_loss_d_pixel, grad_params, stats,
+                        scratch, &g_err);
 }
 
 }  // extern "C"

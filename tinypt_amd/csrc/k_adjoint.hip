@@ -376,9 +376,25 @@ __global__ void __launch_bounds__(TPT_ADJ_BLOCK) __attribute__((amdgpu_waves_per
   }
 }
 
+static int adjoint_blocks(uint64_t n_paths) {
+  return (int)std::max<uint64_t>(std::min<uint64_t>((n_paths + TPT_ADJ_BLOCK - 1) / TPT_ADJ_BLOCK, 8192), 1);
+}
+
+void adjoint_scratch_bytes(const tpt_render_config* cfg, int n_mats, int stack_need, size_t bytes[ADJ_SCRATCH_N]) {
+  const size_t npix = (size_t)cfg->width * cfg->height;
+  const uint64_t n_paths = (uint64_t)npix * (uint64_t)std::max(cfg->spp, 0);
+  const int gcap = stack_need > TPT_WF_STACK_ADJ ? stack_need - TPT_WF_STACK_ADJ : 0;
+  bytes[ADJ_SCRATCH_DLDP] = npix * 12;
+  bytes[ADJ_SCRATCH_GRAD] = (size_t)std::max(n_mats, 1) * 4 * 8;
+  bytes[ADJ_SCRATCH_STATS] = 64;
+  bytes[ADJ_SCRATCH_APM] = (size_t)std::max(n_mats, 1) * 2 * 4;
+  bytes[ADJ_SCRATCH_GSTACK] = gcap > 0 ? (size_t)adjoint_blocks(n_paths) * TPT_ADJ_BLOCK * gcap * 4 : 0;
+}
+
 int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const EnvDev& env, const LightsDev& lights, const MatsDev& mats,
                    const CamF32& cam, const double* mat_param, int stack_need, const tpt_render_config* cfg,
-                   const float* d_loss_d_pixel, double* grad_params, tpt_stats* stats, std::string* err) {
+                   const float* d_loss_d_pixel, double* grad_params, tpt_stats* stats, const AdjScratch& scratch,
+                   std::string* err) {
   auto fail = [&](int code, const std::string& m) {
     *err = m;
     return code;
@@ -386,34 +402,20 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   if (cfg->adaptive_enabled) return fail(TPT_ERR_UNSUPPORTED, "the adjoint replays fixed-spp renders only");
   const size_t npix = (size_t)cfg->width * cfg->height;
   const uint64_t n_paths = (uint64_t)npix * (uint64_t)std::max(cfg->spp, 0);
-  const int blocks = (int)std::min<uint64_t>((n_paths + TPT_ADJ_BLOCK - 1) / TPT_ADJ_BLOCK, 8192);
+  const int blocks = adjoint_blocks(n_paths);
   const int gcap = stack_need > TPT_WF_STACK_ADJ ? stack_need - TPT_WF_STACK_ADJ : 0;
-  void *d_dldp = nullptr, *d_grad = nullptr, *d_stats = nullptr, *d_gs = nullptr, *d_apm = nullptr;
   std::vector<float> apm(2 * (size_t)std::max(mats.n, 1));
   for (int m = 0; m < mats.n; m++) {
     apm[2 * m] = (float)(mat_param[4 * m + 3] + TPT_ADJ_ALPHA_H);
     apm[2 * m + 1] = (float)(mat_param[4 * m + 3] - TPT_ADJ_ALPHA_H);
   }
-  auto cleanup = [&]() {
-    if (d_apm) (void)hipFree(d_apm);
-    if (d_dldp) (void)hipFree(d_dldp);
-    if (d_grad) (void)hipFree(d_grad);
-    if (d_stats) (void)hipFree(d_stats);
-    if (d_gs) (void)hipFree(d_gs);
-  };
-  hipError_t e = hipMalloc(&d_dldp, npix * 12);
-  if (e == hipSuccess) e = hipMalloc(&d_grad, (size_t)mats.n * 4 * 8 + 8);
-  if (e == hipSuccess) e = hipMalloc(&d_stats, 64);
-  if (e == hipSuccess) e = hipMalloc(&d_apm, apm.size() * 4);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_apm, apm.data(), apm.size() * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && gcap > 0) e = hipMalloc(&d_gs, (size_t)std::max(blocks, 1) * TPT_ADJ_BLOCK * gcap * 4);
+  void *d_dldp = scratch.dldp, *d_grad = scratch.grad, *d_stats = scratch.stats, *d_gs = scratch.gstack,
+       *d_apm = scratch.apm;
+  hipError_t e = hipMemcpyAsync(d_apm, apm.data(), apm.size() * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_dldp, d_loss_d_pixel, npix * 12, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_grad, grad_params, (size_t)mats.n * 4 * 8, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemsetAsync(d_stats, 0, 64, s);
-  if (e != hipSuccess) {
-    cleanup();
-    return fail(TPT_ERR_DEVICE, std::string("adjoint setup: ") + hipGetErrorString(e));
-  }
+  if (e != hipSuccess) return fail(TPT_ERR_DEVICE, std::string("adjoint setup: ") + hipGetErrorString(e));
   AdjParams A{};
   A.S = S;
   A.S64 = S64;
@@ -441,7 +443,7 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   (void)hipEventRecord(e0, s);
-  if (n_paths > 0) hipLaunchKernelGGL(k_adjoint, dim3(std::max(blocks, 1)), dim3(TPT_ADJ_BLOCK), 0, s, A);
+  if (n_paths > 0) hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(TPT_ADJ_BLOCK), 0, s, A);
   e = hipGetLastError();
   (void)hipEventRecord(e1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(grad_params, d_grad, (size_t)mats.n * 4 * 8, hipMemcpyDeviceToHost, s);
@@ -452,7 +454,6 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  cleanup();
   if (e != hipSuccess) return fail(TPT_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
   if (stats) {
     stats->paths = h[2];
