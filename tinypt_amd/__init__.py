@@ -218,9 +218,14 @@ class DeviceScene(_SceneTables):
         self.close()
 
     def update_materials(self, mat_param: np.ndarray):
-        """Replace every material's 4 parameters (``tpt_scene_update_materials``)."""
+        """Replace every material's 4 parameters (``tpt_scene_update_materials``); a table equal to
+        the one uploaded last is not uploaded again."""
         p = np.ascontiguousarray(np.asarray(mat_param, dtype=np.float64).reshape(self.n_materials, 4))
+        last = getattr(self, "_last_materials", None)
+        if last is not None and np.array_equal(last, p):
+            return
         _check(self._lib.tpt_scene_update_materials(self.ctx.h, self.h, _dp(p)))
+        self._last_materials = p.copy()
 
     def trace_closest(self, rays: np.ndarray) -> dict:
         """FP32 closest hit (World::hit semantics).  rays (n, 8): o, d, tmin, tmax."""
