@@ -218,6 +218,23 @@ int tpt_render_device(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_con
 int tpt_render_pixels(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cfg, int64_t n,
                       const int32_t* pixels, double* sum_rgb, double* sum_w, double* sum_sq, tpt_stats* stats);
 
+/* Pass sizing of an FP32 wavefront render (host arithmetic; no device, no context).  The
+ * wavefront form keeps pixels x samples paths in flight per pass.  For npix pixels and at most
+ * path_budget paths in flight (the renders use 2^26, halved while device memory is short):
+ * paths_alloc = path slots the render allocates; samples_per_pass / pixels_per_pass = the
+ * fixed-spp passes, or, when cfg->adaptive_enabled, the adaptive pass that starts after `taken`
+ * samples with n_active pixels still sampling (its active list is launched in chunks of
+ * pixels_per_pass pixels).  No pass holds more than paths_alloc paths.  No reference
+ * counterpart: the reference's worker threads take one sample of one pixel at a time
+ * (render.rs:131-191). */
+typedef struct tpt_pass_plan {
+  uint64_t paths_alloc;
+  uint32_t samples_per_pass;
+  uint32_t pixels_per_pass;
+} tpt_pass_plan;
+int tpt_render_plan(const tpt_render_config* cfg, int64_t npix, uint32_t path_budget, uint32_t n_active,
+                    uint32_t taken, tpt_pass_plan* out);
+
 /* Closest hit of n rays (World::hit, world.rs:81-121) in FP32.  rays[n*8]:
  * ox oy oz dx dy dz tmin tmax (d need not be unit length; it is used as given).  Outputs per
  * ray: hit_t (world t, +inf on miss), hit_kind (0 miss, 1 triangle, 2 sphere), hit_inst

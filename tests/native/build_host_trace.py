@@ -1,0 +1,55 @@
+"""Builds the CPU tests' host build of the FP32 traversal (libhost_trace.so, beside this file):
+tpt_trace32.h compiled for the host over the product's host scene build (scene_host.cpp), so
+tests/test_host_traversal.py can hold the traversal against the oracle without a GPU.  Test
+infrastructure: the product library does not contain it.
+
+    python tests/native/build_host_trace.py
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(os.path.dirname(HERE)), "tinypt_amd", "csrc")
+LIB = os.path.join(HERE, "libhost_trace.so")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ARCH = os.environ.get("TPT_OFFLOAD_ARCH", "gfx950")
+
+
+def build(verbose: bool = False) -> str:
+    tmp = LIB + ".tmp"
+    scene_obj = os.path.join(HERE, "scene_host.o")
+    hook_obj = os.path.join(HERE, "host_trace.o")
+    cmds = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC, "-c",
+         os.path.join(CSRC, "scene_host.cpp"), "-o", scene_obj],
+        [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-function", "-Xclang",
+         "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-c", os.path.join(HERE, "host_trace.hip"),
+         "-o", hook_obj],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", hook_obj, scene_obj, "-o", tmp],
+    ]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        r = subprocess.run(c, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("host_trace build failed:\n" + " ".join(c) + "\n" + r.stdout)
+    os.replace(tmp, LIB)
+    os.remove(scene_obj)
+    os.remove(hook_obj)
+    return LIB
+
+
+def is_stale() -> bool:
+    """True when the library is missing or older than a source it is built from."""
+    if not os.path.exists(LIB):
+        return True
+    t = os.path.getmtime(LIB)
+    srcs = [os.path.join(HERE, "host_trace.hip")] + [os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                                                      if f.endswith((".h", ".cpp"))]
+    return any(os.path.getmtime(s) > t for s in srcs)
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,6 +1,6 @@
 // host_trace.hip -- test hook: the FP32 traversal (tpt_trace32.h) compiled for the host and run
 // over the product's host scene build, so CPU tests can compare it with the oracle's World::hit
-// without a GPU.  Not part of the product library; built next to it by tinypt_amd/build.py.
+// without a GPU.  Not part of the product library; built by tests/native/build_host_trace.py.
 #include <cmath>
 #include <vector>
 

@@ -9,6 +9,7 @@ ties and rays within 1e-5 of an edge (the oracle itself flips on a nudged copy).
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -22,8 +23,11 @@ LIB = os.path.join(os.path.dirname(__file__), "native", "libhost_trace.so")
 
 
 def _lib():
-    if not os.path.exists(LIB):
-        pytest.fail("tests/native/libhost_trace.so is not built (run __graft_entry__.build())")
+    """The host build of the traversal, (re)built by tests/native/build_host_trace.py when a source is newer."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "native"))
+    import build_host_trace
+    if build_host_trace.is_stale():
+        build_host_trace.build()
     return C.CDLL(LIB)
 
 

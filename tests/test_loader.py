@@ -194,6 +194,18 @@ def test_envmap_hdr_file_loads(tmp_path, rle, w):
     np.testing.assert_array_equal(sc.env.data, dec * 2.0)
 
 
+@pytest.mark.parametrize("res", ["-Y 2251799813685248 +X 8192", "-Y 4294967296 +X 4294967296",
+                                 "-Y 0 +X 16", "-Y 16 +X 0", "-Y 16385 +X 16385"])
+def test_envmap_hdr_malformed_size_is_rejected(tmp_path, res):
+    """Resolution lines whose texel count is zero, above 2^28 or wraps 64 bits: the map is
+    refused with a warning (the scene loads with the black default) instead of being read."""
+    (tmp_path / "env.hdr").write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + res.encode() + b"\n" + bytes(64))
+    warns = []
+    sc = _load(tmp_path, """<scene version="3.0.0">
+      <emitter type="envmap"><string name="filename" value="env.hdr"/></emitter></scene>""", warnings=warns)
+    assert not sc.env.data.any() and any("env.hdr" in w for w in warns)
+
+
 def test_exr_envmap_is_skipped_with_a_warning(tmp_path):
     warns = []
     sc = _load(tmp_path, """<scene version="3.0.0">

@@ -102,6 +102,14 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class PassPlan(C.Structure):
+    _fields_ = [
+        ("paths_alloc", C.c_uint64),
+        ("samples_per_pass", C.c_uint32),
+        ("pixels_per_pass", C.c_uint32),
+    ]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [
         ("n_tris", C.c_int64),
@@ -134,6 +142,8 @@ ENTRY_POINTS = {
     "tpt_render": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), _dp, _dp, _dp, C.POINTER(Stats)]),
     "tpt_render_device": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), _vp, _vp, _vp, C.POINTER(Stats)]),
     "tpt_trace_closest": (C.c_int, [_vp, _vp, C.c_int64, _fp, _fp, _ip, _ip, _ip]),
+    "tpt_render_plan": (C.c_int, [C.POINTER(RenderConfigC), C.c_int64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.POINTER(PassPlan)]),
     "tpt_render_adjoint": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), _fp, _dp, C.POINTER(Stats)]),
     "tpt_scene_update_materials": (C.c_int, [_vp, _vp, _dp]),
     "tpt_scene_build_host": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(_vp)]),

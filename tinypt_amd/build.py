@@ -57,13 +57,8 @@ def _run(cmd, verbose):
     return r.stdout
 
 
-TEST_HOOK_SRC = os.path.join(os.path.dirname(HERE), "tests", "native", "host_trace.hip")
-TEST_HOOK_LIB = os.path.join(os.path.dirname(HERE), "tests", "native", "libhost_trace.so")
-
-
 def build(verbose: bool = False, jobs: int = 4) -> str:
-    """Compile every source and link the shared library; returns its path.  Also links the
-    CPU tests' host build of the FP32 traversal (tests/native), a separate library."""
+    """Compile every source and link the shared library; returns its path."""
     os.makedirs(OBJDIR, exist_ok=True)
     jobs_list = []
     for src, flags in SOURCES:
@@ -78,13 +73,6 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
-    if os.path.exists(TEST_HOOK_SRC):
-        hook_obj = os.path.join(OBJDIR, "host_trace.hip.o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", TEST_HOOK_SRC, "-o", hook_obj], verbose)
-        scene_obj = os.path.join(OBJDIR, "scene_host.cpp.o")
-        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", TEST_HOOK_LIB + ".tmp", hook_obj, scene_obj],
-             verbose)
-        os.replace(TEST_HOOK_LIB + ".tmp", TEST_HOOK_LIB)
     return LIB
 
 

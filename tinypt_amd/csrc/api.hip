@@ -9,6 +9,7 @@
 
 #include "../../include/tinypt_gpu.h"
 #include "adjoint.h"
+#include "pass_plan.h"
 #include "plain64.h"
 #include "scene_host.h"
 #include "wavefront.h"
@@ -501,6 +502,7 @@ struct EventTimer {
 int wf_bounces(tpt_ctx* c, const WfParams& W, const PassArgs& A, EventTimer& tt, uint64_t* launches) {
   hipStream_t s = c->stream;
   const uint32_t P = A.n_paths;
+  if ((size_t)P > c->ps_cap) return fail(TPT_ERR_INTERNAL, "a pass holds more paths than were allocated");
   if (W.max_bounces == 0) HIPC(wf_launch_clear_radiance(W, P, s));
   int cur = 0;
   for (int b = 0; b < W.max_bounces; b++) {
@@ -536,17 +538,10 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
       while (target > (1u << 20) && (size_t)target * per_path > free_b / 2) target >>= 1;
     }
   }
-  uint32_t k = cfg->samples_per_pass > 0 ? (uint32_t)cfg->samples_per_pass : std::max(1u, target / std::max(npix, 1u));
-  k = std::max(1u, std::min(k, std::max(spp, 1u)));
-  uint32_t pix_chunk = npix;
-  if ((uint64_t)npix * k > (uint64_t)target * 2) {  // very large images: split the pixels
-    k = 1;
-    pix_chunk = std::min(npix, target);
-  }
+  const PassPlan plan = plan_render(npix, spp, cfg->samples_per_pass, cfg->adaptive_enabled != 0, target);
+  const uint32_t k = plan.k, pix_chunk = plan.pix_chunk;
   const int gcap = needed_gstack(sc->host);
-  size_t P = (size_t)pix_chunk * k;
-  // adaptive passes give each active pixel up to 64 samples, within the same in-flight budget
-  if (cfg->adaptive_enabled) P = std::max(P, (size_t)std::min<uint64_t>((uint64_t)npix * 64, (uint64_t)target));
+  const size_t P = (size_t)plan.paths_alloc;
   int rc = ensure_paths(c, P, gcap);
   if (rc) return rc;
   WfParams W = make_wf(c, sc, cfg, gcap);
@@ -586,20 +581,22 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
     int32_t* act_in = c->act0.as<int32_t>();
     int32_t* act_out = c->act1.as<int32_t>();
     while (n_active > 0 && taken < spp) {
-      uint32_t kk = taken < min_spp ? min_spp - taken : 1;
-      uint32_t fill = (uint32_t)std::max<size_t>(1, P / n_active);
-      kk = std::max(kk, std::min<uint32_t>(fill, 64));
-      kk = std::min(kk, spp - taken);
-      kk = std::min<uint32_t>(kk, (uint32_t)std::max<size_t>(1, P / n_active));
-      PassArgs A{n_active * kk, n_active, 0, 0, act_in, c->nsamp.as<uint32_t>(), pix_map};
-      rc = wf_bounces(c, W, A, tt, &launches);
-      if (rc) return rc;
-      traced += (uint64_t)n_active * kk;
+      uint32_t kk = 1, chunk = 1;
+      plan_adaptive_pass(n_active, taken, min_spp, spp, P, &kk, &chunk);
       uint32_t* n_out = W.ps.counters + TPT_CNT_ACTIVE;
       HIPC(hipMemsetAsync(n_out, 0, 4, s));
-      AdaptiveArgs B{n_active, kk, min_spp, spp, cfg->adaptive_threshold, act_in, act_out, n_out,
-                     c->nsamp.as<uint32_t>(), c->welford.as<double>(), acc, acc_w, acc_sq};
-      HIPC(wf_launch_accumulate_adaptive(W, B, s));
+      // the active list in chunks of at most P / kk pixels; survivors of every chunk are
+      // appended to act_out (a pixel's samples do not depend on its position in the list)
+      for (uint32_t c0 = 0; c0 < n_active; c0 += chunk) {
+        const uint32_t nc = std::min(chunk, n_active - c0);
+        PassArgs A{nc * kk, nc, 0, 0, act_in + c0, c->nsamp.as<uint32_t>(), pix_map};
+        rc = wf_bounces(c, W, A, tt, &launches);
+        if (rc) return rc;
+        traced += (uint64_t)nc * kk;
+        AdaptiveArgs B{nc, kk, min_spp, spp, cfg->adaptive_threshold, act_in + c0, act_out, n_out,
+                       c->nsamp.as<uint32_t>(), c->welford.as<double>(), acc, acc_w, acc_sq};
+        HIPC(wf_launch_accumulate_adaptive(W, B, s));
+      }
       HIPC(hipMemcpyAsync(&n_active, n_out, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipStreamSynchronize(s));
       taken += kk;
@@ -768,6 +765,28 @@ int tpt_render(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_config* cf
   HIPC(hipSetDevice(ctx->device));
   return render_to_host(ctx, scene, cfg, (uint32_t)cfg->width * (uint32_t)cfg->height, nullptr, acc_rgb, acc_w, acc_sq,
                         stats);
+}
+
+int tpt_render_plan(const tpt_render_config* cfg, int64_t npix, uint32_t path_budget, uint32_t n_active, uint32_t taken,
+                    tpt_pass_plan* out) {
+  if (!out) return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if (npix <= 0 || npix > (int64_t)1 << 28) return fail(TPT_ERR_INVALID_ARGUMENT, "pixel count out of range");
+  const uint32_t spp = (uint32_t)cfg->spp;
+  const PassPlan plan = plan_render((uint32_t)npix, spp, cfg->samples_per_pass, cfg->adaptive_enabled != 0,
+                                    path_budget);
+  out->paths_alloc = plan.paths_alloc;
+  out->samples_per_pass = plan.k;
+  out->pixels_per_pass = plan.pix_chunk;
+  if (cfg->adaptive_enabled) {
+    if (n_active == 0 || n_active > npix) return fail(TPT_ERR_INVALID_ARGUMENT, "n_active must be in 1..npix");
+    const uint32_t max_spp = std::max(spp, 1u);
+    const uint32_t min_spp = std::min((uint32_t)std::max(cfg->adaptive_min_spp, 1), max_spp);
+    plan_adaptive_pass(n_active, taken, min_spp, spp, plan.paths_alloc, &out->samples_per_pass,
+                       &out->pixels_per_pass);
+  }
+  return TPT_OK;
 }
 
 int tpt_trace_closest(tpt_ctx* ctx, const tpt_scene* sc, int64_t n, const float* rays, float* hit_t, int32_t* hit_kind,

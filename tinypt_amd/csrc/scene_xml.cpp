@@ -519,7 +519,11 @@ bool load_hdr(const std::string& path, int* w_out, int* h_out, std::vector<doubl
   if (!parse_usize(p3, &ww)) { *err = "invalid width"; return false; }
   const char ys = p0[0], xs = p2[0];
   const size_t w = (size_t)ww, h = (size_t)hh;
-  if (w * h > ((size_t)1 << 28)) { *err = "image too large"; return false; }
+  // an empty image has no texel to sample; the product is checked without wrapping (both
+  // factors may be up to 19 digits long)
+  if (w == 0 || h == 0) { *err = "empty HDR image"; return false; }
+  size_t texels = 0;
+  if (__builtin_mul_overflow(w, h, &texels) || texels > ((size_t)1 << 28)) { *err = "image too large"; return false; }
   rgb->assign(3 * w * h, 0.0);
   std::vector<uint8_t> scan(4 * w);
   auto need = [&](size_t n) -> bool {

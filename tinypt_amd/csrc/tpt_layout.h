@@ -1,9 +1,10 @@
 // tpt_layout.h -- device memory layout shared by the host scene build and the HIP kernels.
 //
-// FP32 kernels read compressed two-child nodes (Node2, 64 B: both children's boxes and refs in
-// one node, so a traversal step is one 64 B fetch and each box is tested once), triangles as
-// v0/e1/e2 (48 B), and per-instance world->object rows.  The plain FP64 kernel reads the
-// reference's own layout (bvh.rs:24-35 nodes, geometry.rs triangles, transform.rs matrices).
+// FP32 kernels read four-wide nodes (Node4, 128 B: up to four child boxes and refs in one node,
+// so a traversal step is one 128 B fetch and each child box is tested once), triangles as
+// v0/e1/e2 (48 B), and per-instance world->object rows.  The host builds binary Node2 trees
+// first and collapses them into Node4.  The plain FP64 kernel reads the reference's own layout
+// (bvh.rs:24-35 nodes, geometry.rs triangles, transform.rs matrices).
 #pragma once
 
 #include <cstdint>

@@ -7,8 +7,8 @@
 // hit when t <= tmax (geometry.rs:324-329), so whether the light shadows its own sample is
 // decided by the last bits of the f64 computation -- about 30% of samples are self-occluded on
 // a rectangle light.  FP32 rounding gives a very different rate.  The FP32 kernels therefore
-// trace their shadow ray with the light's own surface excluded (tmax pulled in by 1e-5
-// relative) and take the self-occlusion decision from this function, which repeats the
+// trace their shadow ray with the light's own surface excluded (tmax pulled in by 2e-4
+// relative, light_shadow_tmax32) and take the self-occlusion decision from this function, which repeats the
 // reference's f64 operations in the reference's order on the FP32 hit point and uniforms.
 // Contraction is off inside the function so each operation rounds as the reference's does.
 #pragma once
