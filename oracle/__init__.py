@@ -67,6 +67,9 @@ def lib():
             "orc_triangle_hit": (C.c_int, [_dp, _dp, C.c_double, C.c_double, _dp]),
             "orc_aabb_hit": (C.c_int, [_dp, _dp, C.c_double, C.c_double, _dp]),
             "orc_camera_rays": (None, [C.POINTER(_abi.CameraDesc), C.c_uint64, C.c_int64, _dp, _dp]),
+            "orc_transform_apply": (None, [_dp, C.c_int32, C.c_int64, _dp, _dp]),
+            "orc_light_shadow": (None, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), _dp, _dp, _dp,
+                                        C.POINTER(C.c_int32)]),
             "orc_material_sample": (None, [C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_int64, _dp]),
             "orc_material_eval": (None, [C.c_int, _dp, _dp, _dp, _dp, _dp]),
             "orc_ggx_vndf": (None, [_dp, C.c_double, C.c_double, C.c_double, _dp]),
@@ -147,6 +150,19 @@ def camera_rays(cam_desc: _abi.CameraDesc, seed: int, sxy) -> np.ndarray:
     sxy = _f64(sxy).reshape(-1, 2)
     out = np.zeros((len(sxy), 7))
     lib().orc_camera_rays(C.byref(cam_desc), seed, len(sxy), _d(sxy), _d(out))
+    return out
+
+
+# ---------------------------------------------------------------- transform.rs
+TRANSFORM_OPS = {"apply_point": 0, "apply_point_inv": 1, "apply_vec_inv": 2, "apply_normal": 3}
+
+
+def transform_apply(xf12, op: str, vecs) -> np.ndarray:
+    """The oracle's Transform (transform.rs:24-93) built from xf12 = A (row-major) + t, applied
+    to each row of vecs (n x 3) by the named method."""
+    v = _f64(np.asarray(vecs, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros_like(v)
+    lib().orc_transform_apply(_d(_f64(xf12)), TRANSFORM_OPS[op], len(v), _d(v), _d(out))
     return out
 
 

@@ -954,12 +954,23 @@ struct Scene {
     light_total = total;
   }
   bool sample_surface(const LightInfo& L, double time, Rng& rng, V3* pos, V3* nrm, double* area) const {
-    // world.rs:242-272
+    // world.rs:242-272: a sphere light draws (u, v) first; a triangle light only once its
+    // world vertices exist
+    if (L.kind == 0) {
+      if (L.idx < 0 || (size_t)L.idx >= spheres.size()) return false;
+    } else {
+      V3 a, b, c;
+      if (!tri_world_verts(L.mesh_id, L.tri_id, L.inst_id, time, &a, &b, &c)) return false;
+    }
+    double u = rng.next_f64();
+    double v = rng.next_f64();
+    return surface_at(L, time, u, v, pos, nrm, area);
+  }
+  // the point of light L at surface uniforms (u, v) (world.rs:244-269)
+  bool surface_at(const LightInfo& L, double time, double u, double v, V3* pos, V3* nrm, double* area) const {
     if (L.kind == 0) {
       if (L.idx < 0 || (size_t)L.idx >= spheres.size()) return false;
       const Sphere& s = spheres[(size_t)L.idx];
-      double u = rng.next_f64();
-      double v = rng.next_f64();
       double z = 1.0 - 2.0 * u;
       double r = std::sqrt(fmx(1.0 - z * z, 0.0));
       double phi = TAU * v;
@@ -971,8 +982,6 @@ struct Scene {
     }
     V3 a, b, c;
     if (!tri_world_verts(L.mesh_id, L.tri_id, L.inst_id, time, &a, &b, &c)) return false;
-    double u = rng.next_f64();
-    double v = rng.next_f64();
     double su = std::sqrt(u);
     double b0 = 1.0 - su;
     double b1 = v * su;
@@ -1392,6 +1401,61 @@ int orc_aabb_hit(const double* box, const double* ray, double tmin, double tmax,
   Ray rr{v3(ray[0], ray[1], ray[2]), v3(ray[3], ray[4], ray[5]), ray[6]};
   V3 inv = v3(1.0 / rr.d.x, 1.0 / rr.d.y, 1.0 / rr.d.z);
   return b.range_inv(rr, inv, tmin, tmax, t0) ? 1 : 0;
+}
+
+// ---- the area-light shadow test of nee_area_light (integrator.rs:178-213) for light `light[i]`
+// sampled at uniforms uv[2i..2i+1] from shading point p[3i..3i+2] at ray time time[i]
+// (world.rs:242-272): out[i] = 0 unoccluded, 1 blocked by the light's own primitive, 2 blocked by
+// other geometry, 3 blocked by another triangle of the light's instance, -1 no shadow ray
+// (dist <= 1e-6 or no surface point)
+void orc_light_shadow(void* s, int64_t n, const int32_t* light, const double* p, const double* uv, const double* time,
+                      int32_t* out) {
+  const Scene& sc = *(Scene*)s;
+  for (int64_t i = 0; i < n; i++) {
+    out[i] = -1;
+    if (light[i] < 0 || (size_t)light[i] >= sc.lights.size()) continue;
+    const LightInfo& L = sc.lights[(size_t)light[i]];
+    V3 pos, nrm;
+    double area;
+    if (!sc.surface_at(L, time[i], uv[2 * i], uv[2 * i + 1], &pos, &nrm, &area)) continue;
+    const V3 hp = v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    V3 to_light = pos - hp;
+    double dist = std::sqrt(dot(to_light, to_light));
+    if (dist <= 1e-6) continue;
+    V3 wi = to_light / dist;
+    Ray shadow{hp + RAY_EPSILON * wi, wi, time[i]};
+    double tmax = fmx(dist - RAY_EPSILON, RAY_EPSILON);
+    Hit h;
+    if (!sc.hit(shadow, RAY_EPSILON, tmax, &h)) {
+      out[i] = 0;
+      continue;
+    }
+    if (L.kind == 0)
+      out[i] = h.kind == HIT_SPHERE && h.inst == L.idx ? 1 : 2;
+    else if (h.kind == HIT_TRI && h.inst == L.inst_id)
+      out[i] = h.prim == L.tri_id ? 1 : 3;
+    else
+      out[i] = 2;
+  }
+}
+
+// ---- transform (transform.rs:24-93).  xf[12] = A row-major, t; op 0 apply_point, 1 apply_point_inv,
+// 2 apply_vec_inv, 3 apply_normal; n vectors in[3n] -> out[3n]
+void orc_transform_apply(const double* xf, int32_t op, int64_t n, const double* in, double* out) {
+  Mat3 a;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) a.m[r][c] = xf[3 * r + c];
+  Transform x = Transform::from_affine(a, v3(xf[9], xf[10], xf[11]));
+  for (int64_t i = 0; i < n; i++) {
+    V3 v = v3(in[3 * i], in[3 * i + 1], in[3 * i + 2]), r;
+    switch (op) {
+      case 0: r = x.apply_point(v); break;
+      case 1: r = x.apply_point_inv(v); break;
+      case 2: r = x.apply_vec_inv(v); break;
+      default: r = x.apply_normal(v); break;
+    }
+    out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+  }
 }
 
 // ---- camera (ray.rs)
