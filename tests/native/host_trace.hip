@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "scene_host.h"
+#include "tpt_selfocc64.h"
 #include "tpt_trace32.h"
 
 using namespace tpt;
@@ -77,6 +78,62 @@ extern "C" int tpt_test_instance_box(int64_t n_tris, const double* tri_v, const 
   for (int k = 0; k < 3; k++) {
     box_cut[k] = c.mn[k]; box_cut[3 + k] = c.mx[k];
     box_root[k] = r.mn[k]; box_root[3 + k] = r.mx[k];
+  }
+  return 0;
+}
+
+// The FP32 kernels' light self-occlusion decision for n light samples, with the inputs the
+// kernels give it: each FP32 ray rays[8i..] (o, d, tmin, tmax) is traced by the FP32 traversal,
+// its shading point rebuilt in FP64 (hit_point64), and light light[i] tested at the 53-bit
+// uniforms uv[2i..].  out[i] = -1 when the ray misses, else 1 when the sampled light shadows its
+// own sample, 0 when not.
+extern "C" int tpt_test_selfocc(const tpt_scene_desc* d, int64_t n, const float* rays, const int32_t* light,
+                                const double* uv, int32_t* out) {
+  HostScene H;
+  if (!H.build(d)) return -1;
+  SceneF32 S{};
+  S.nodes = H.nodes4.data();
+  S.tris = H.tris32.data();
+  S.tri_mat = H.tri_mat32.data();
+  S.tri_orig = H.tri_orig.data();
+  S.tri_inst = H.flat ? H.tri_inst32.data() : nullptr;
+  S.inst = H.inst32.data();
+  S.tlas_items = H.tlas_items.data();
+  S.sph = H.sph.data();
+  S.sph_mat = H.sph_mat.data();
+  S.tlas_root = H.tlas_root;
+  S.n_spheres = (int32_t)H.sph_mat.size();
+  S.flat_inst = H.flat ? H.flat_inst : -1;
+  SceneF64 S64{};
+  S64.nodes = H.ref_nodes.data();
+  S64.indices = H.ref_indices.data();
+  S64.tris = H.tri_v.data();
+  S64.tri_mat = H.tri_mat.data();
+  S64.meshes = H.meshes.data();
+  S64.inst = H.inst64.data();
+  S64.sph = H.sph.data();
+  S64.sph_mat = H.sph_mat.data();
+  S64.n_inst = (int32_t)H.inst64.size();
+  S64.n_meshes = (int32_t)H.meshes.size();
+  S64.n_spheres = (int32_t)H.sph_mat.size();
+  LightsDev Ls{};
+  Ls.n = (int32_t)H.lights.size();
+  Ls.l64 = H.lights.data();
+  Ls.cdf64 = H.light_cdf.data();
+  std::vector<int32_t> stack((size_t)H.stack4 + 8);
+  LaneStack st{stack.data(), stack.data() + 4, 1, 4};
+  for (int64_t i = 0; i < n; i++) {
+    if (light[i] < 0 || light[i] >= Ls.n) return -2;
+    const float* r = rays + 8 * i;
+    const vec3<float> o = mk(r[0], r[1], r[2]), dd = mk(r[3], r[4], r[5]);
+    HitF32 h;
+    if (!trace_f32<false>(S, o, dd, r[6], r[7], &h, st)) {
+      out[i] = -1;
+      continue;
+    }
+    const vec3<float> p32 = o + dd * h.t;
+    const D3 p = hit_point64(S64, S.tri_orig, h.inst, h.prim, o, dd, p32);
+    out[i] = light_self_occluded64(S64, Ls, light[i], p, uv[2 * i], uv[2 * i + 1]) ? 1 : 0;
   }
   return 0;
 }

@@ -454,7 +454,7 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
     ps.sh_c[q] = c->sh_c[q].as<float4>();
   }
   ps.sh_l = c->sh_l.as<float4>();
-  ps.sh_lv = c->sh_lv.as<float2>();
+  ps.sh_lv = c->sh_lv.as<uint64_t>();
   ps.shl_keep = c->shl_keep.as<uint32_t>();
   ps.counters = c->counters.as<uint32_t>();
   ps.stats = c->stats.as<uint64_t>();
@@ -513,7 +513,7 @@ int wf_bounces(tpt_ctx* c, const WfParams& W, const PassArgs& A, EventTimer& tt,
     (*launches)++;
     HIPC(wf_launch_shade(W, cur, gen, P, s));
     if (W.lights.n > 0) {
-      HIPC(wf_launch_light_selfocc(W, P, s));
+      HIPC(wf_launch_light_selfocc(W, cur, gen, P, s));
       HIPC(wf_launch_shadow(W, 1, P, s));
     }
     if (W.env.kind == TPT_ENV_IMAGE && !W.env.black) HIPC(wf_launch_shadow(W, 0, P, s));

@@ -240,7 +240,8 @@ __device__ __forceinline__ f3 replay(const int MODE, const AdjParams& A, uint32_
             float w = mis_weight(ls.pdf, pb);
             f3 c = had(had(T, f), ls.emit) * (cs * w / ls.pdf);
             if ((c.x != 0.f || c.y != 0.f || c.z != 0.f) &&
-                !light_self_occluded64(A.S64, A.lights, ls.idx, p.x, p.y, p.z, ls.u, ls.v)) {
+                !light_self_occluded64(A.S64, A.lights, ls.idx, hit_point64(A.S64, A.S.tri_orig, h.inst, h.prim, o, d, p),
+                                       ls.u, ls.v)) {
               float tmax = light_shadow_tmax32(dist);
               HitF32 hs;
               (*n_sh)++;

@@ -192,7 +192,7 @@ k_shade(WfParams W, int cur, PassArgs A) {
     // ---- phase 1: miss, Russian roulette, emission, next-event estimation
     float4 sh_c0 = make_float4(0.f, 0.f, 0.f, 0.f), sh_c1 = sh_c0;
     float4 sh_o0 = sh_c0, sh_d0 = sh_c0, sh_d1 = sh_c0, sh_l = sh_c0;  // 0: environment, 1: light
-    float2 sh_lv = make_float2(0.f, 0.f);
+    uint64_t sh_lv = 0;
     bool go = false;  // hit a scattering surface: sample the BSDF in phase 2
     f3 p = mk(0.f, 0.f, 0.f), ng = p;
     int mat = 0, kind = 0;
@@ -247,6 +247,7 @@ k_shade(WfParams W, int cur, PassArgs A) {
               }
             }
             LightSample32 ls;  // nee_area_light (integrator.rs:178-213)
+            const uint64_t light_state = rng.p.state;
             if (sample_light32(W.lights, rng, p, &ls)) {
               f3 to_light = ls.pos - p;
               float dist = sqrtf(dot(to_light, to_light));
@@ -262,8 +263,8 @@ k_shade(WfParams W, int cur, PassArgs A) {
                     // the shadow kernel rebuilds origin p + eps wi and tmax from p and dist
                     sh_d1 = make_float4(wi.x, wi.y, wi.z, __int_as_float(id));
                     sh_c1 = make_float4(c.x, c.y, c.z, dist);
-                    sh_l = make_float4(p.x, p.y, p.z, ls.u);  // also for k_light_selfocc
-                    sh_lv = make_float2(ls.v, __int_as_float(ls.idx));
+                    sh_l = make_float4(p.x, p.y, p.z, __int_as_float((int32_t)i));  // also for k_light_selfocc
+                    sh_lv = light_state;
                     has_light = true;
                   }
                 }
@@ -313,12 +314,18 @@ k_shade(WfParams W, int cur, PassArgs A) {
 
 // The reference's light shadow ray can be blocked by the light itself; that decision is taken
 // in FP64 (tpt_selfocc64.h) here, in a kernel of its own so the FP64 registers do not weigh on
-// k_shade.  The slots of the rays that remain go densely into the SHK queue, so k_shadow's
-// waves carry no lanes for dropped rays (about 30% on a rectangle light).
-__global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
+// k_shade.  Its inputs are rebuilt with the reference's precision: the shading point by the
+// FP64 World::hit of the path's FP32 ray on the primitive it hit (the ray from the path queue,
+// or, on the first bounce, the camera ray generated again), the light and its 53-bit uniforms
+// drawn again from the stream state saved before the light sample.  The slots of the rays that
+// remain go densely into the SHK queue, so k_shadow's waves carry no lanes for dropped rays
+// (about 30% on a rectangle light).
+template <bool GEN>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W, int cur, PassArgs A) {
   __shared__ uint32_t s_slot[8];
-  QueueMap qm;
+  QueueMap qm, qp;
   qm.load(W.ps.counters, TPT_CNT_SHL, W.ps.seg_cap);
+  if constexpr (!GEN) qp.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
   const uint32_t n = qm.n;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
@@ -327,8 +334,23 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W) {
     if (j < n) {
       slot = qm.slot(j);
       const float4 l = W.ps.sh_l[slot];
-      const float2 lv = W.ps.sh_lv[slot];
-      keep = !light_self_occluded64(W.S64, W.lights, __float_as_int(lv.y), l.x, l.y, l.z, l.w, lv.x);
+      const uint32_t pos = (uint32_t)__float_as_int(l.w);  // the path's dense position this bounce
+      f3 o, d;
+      if constexpr (GEN) {
+        RngF32 cam(1);
+        gen_path(W, A, pos, &cam, &o, &d);
+      } else {
+        const uint32_t ps = qp.slot(pos);
+        o = f3of(W.ps.o[cur][ps]);
+        d = f3of(W.ps.d[cur][ps]);
+      }
+      const HitRec hr = W.ps.hit[pos];
+      const D3 p64 = hit_point64(W.S64, W.S.tri_orig, hr.inst, hr.prim, o, d, f3of(l));
+      RngF32 rng(1);
+      rng.p.state = W.ps.sh_lv[slot];
+      const int idx = light_pick32(W.lights, rng);
+      const double u = rng.uni53(), v = rng.uni53();
+      keep = !light_self_occluded64(W.S64, W.lights, idx, p64, u, v);
     }
     const uint32_t k = seg_slot(keep, W.ps, TPT_CNT_SHK, s_slot);  // every thread of the block
     if (keep && k != UINT32_MAX) W.ps.shl_keep[k] = slot;
@@ -517,8 +539,14 @@ hipError_t wf_launch_shade(const WfParams& W, int cur, const PassArgs* gen, uint
   }
   return hipGetLastError();
 }
-hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_light_selfocc, grid_of(max_n, (const void*)k_light_selfocc), dim3(TPT_WF_BLOCK), 0, s, W);
+hipError_t wf_launch_light_selfocc(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s) {
+  const PassArgs A = gen ? *gen : PassArgs{};
+  if (gen)
+    hipLaunchKernelGGL(k_light_selfocc<true>, grid_of(max_n, (const void*)k_light_selfocc<true>), dim3(TPT_WF_BLOCK), 0, s,
+                       W, cur, A);
+  else
+    hipLaunchKernelGGL(k_light_selfocc<false>, grid_of(max_n, (const void*)k_light_selfocc<false>), dim3(TPT_WF_BLOCK), 0,
+                       s, W, cur, A);
   return hipGetLastError();
 }
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s) {

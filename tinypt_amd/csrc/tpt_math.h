@@ -107,6 +107,13 @@ struct RngF32 {
   Pcg32 p;
   TPT_HD explicit RngF32(uint64_t seed) : p(seed) {}
   TPT_HD float uni() { return (float)(p.next_u32() >> 8) * (1.0f / 16777216.0f); }
+  // Rng::next_f64's 53-bit uniform (two outputs), where a decision taken in FP64 needs the
+  // full-precision draw (the light self-occlusion test)
+  TPT_HD double uni53() {
+    uint64_t hi = ((uint64_t)p.next_u32()) << 21;
+    uint64_t lo = ((uint64_t)p.next_u32()) & ((1ull << 21) - 1);
+    return (double)(hi | lo) * (1.0 / 9007199254740992.0);
+  }
 };
 
 }  // namespace tpt

@@ -85,18 +85,26 @@ struct LightSample32 {
   f3 pos, emit;
   float pdf;
   int mat;
-  int idx;     // light drawn
-  float u, v;  // its surface uniforms
+  int idx;        // light drawn
+  double u, v;    // its surface uniforms (53-bit draws; the FP32 position uses them rounded)
 };
+// The light a sample draws (world.rs:173-176): its index from one uniform and the CDF.
+__device__ __forceinline__ int light_pick32(const LightsDev& Ls, RngF32& rng) {
+  float r = rng.uni() * Ls.total32;
+  return min(cdf_search(Ls.cdf32, Ls.n, r), Ls.n - 1);
+}
+// Light sample of world.rs:171-209.  The surface uniforms are 53-bit draws, like the
+// reference's: the FP64 self-occlusion decision (tpt_selfocc64.h) depends on their low bits,
+// and regenerates them from the stream state before this call (light_pick32, then two uni53).
 __device__ __forceinline__ bool sample_light32(const LightsDev& Ls, RngF32& rng, f3 p, LightSample32* out) {
   if (!(Ls.total32 > 0.f) || Ls.n == 0) return false;
-  float r = rng.uni() * Ls.total32;
-  int idx = min(cdf_search(Ls.cdf32, Ls.n, r), Ls.n - 1);
+  const int idx = light_pick32(Ls, rng);
   const LightF32& L = Ls.l32[idx];
   float pdf_select = L.emit[3] / Ls.total32;
   f3 pos, nrm;
   float area;
-  float u = rng.uni(), v = rng.uni();
+  const double u53 = rng.uni53(), v53 = rng.uni53();
+  const float u = (float)u53, v = (float)v53;
   if (L.kind == 0) {
     float z = 1.f - 2.f * u;
     float rr = sqrtf(fmaxf(1.f - z * z, 0.f));
@@ -129,8 +137,8 @@ __device__ __forceinline__ bool sample_light32(const LightsDev& Ls, RngF32& rng,
   out->pdf = pdf;
   out->mat = L.mat;
   out->idx = idx;
-  out->u = u;
-  out->v = v;
+  out->u = u53;
+  out->v = v53;
   return true;
 }
 

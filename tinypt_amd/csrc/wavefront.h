@@ -66,8 +66,9 @@ struct PathState {
   float4* sh_d[2];  // direction, path id (int bits)
   float4* sh_c[2];  // contribution if unoccluded; w: 1 (environment) or the distance to the light
                     // sample (light; origin p + eps wi and tmax follow from it)
-  float4* sh_l;     // light queue: shading point xyz, light-sample u
-  float2* sh_lv;    // light queue: light-sample v, light index (int bits)
+  float4* sh_l;     // light queue: shading point xyz, dense position of the path in its queue (int bits)
+  uint64_t* sh_lv;  // light queue: random stream state before the light sample (its light and
+                    // 53-bit uniforms are drawn again from it for the self-occlusion test)
   uint32_t* shl_keep;  // segmented queue of light-queue slots whose ray is traced (not self-occluded)
   uint32_t* counters;
   uint64_t* stats;  // per segment g: [g * TPT_STAT_STRIDE + k], k = 0 extension rays, 1 shadow rays, 2 paths
@@ -155,7 +156,9 @@ hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s
 // gen: the pass's paths, for the first bounce (its kernels generate the camera rays); null after
 hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shade(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
-hipError_t wf_launch_light_selfocc(const WfParams& W, uint32_t max_n, hipStream_t s);
+// cur: the queue of paths shaded this bounce; gen: the pass's paths on the first bounce (their
+// rays are generated again from their ids), null after
+hipError_t wf_launch_light_selfocc(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s);
 hipError_t wf_launch_accumulate(const WfParams& W, const AccumArgs& A, hipStream_t s);
 hipError_t wf_launch_accumulate_adaptive(const WfParams& W, const AdaptiveArgs& A, hipStream_t s);
