@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TPT_ABI_VERSION 1
+#define TPT_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------------------------ */
 typedef enum tpt_status {
@@ -133,6 +133,9 @@ typedef struct tpt_stats {
   double trace_kernel_ms;    /* summed device time of the extension traversal kernel            */
   uint64_t trace_launches;   /* launches of the extension traversal kernel                      */
   uint64_t trace_rays;       /* rays processed by those launches                                */
+  uint64_t light_samples;    /* area-light samples with a non-zero contribution (FP32 form)     */
+  uint64_t light_self_occluded; /* of those, the samples whose light shadows its own surface
+                                   point (integrator.rs:178-213, the reference's f64 decision)  */
 } tpt_stats;
 
 /* ---- opaque handles ------------------------------------------------------------------------ */

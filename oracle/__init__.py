@@ -27,7 +27,8 @@ _u64p = C.POINTER(C.c_uint64)
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("aabb", "tri", "sph", "inst", "rays_ext", "rays_shadow", "paths", "ext_bounces",
-                                                  "aabb_ext", "tri_ext", "sph_ext", "inst_ext")]
+                                                  "aabb_ext", "tri_ext", "sph_ext", "inst_ext", "light_samples",
+                                                  "light_self")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -68,6 +69,8 @@ def lib():
             "orc_aabb_hit": (C.c_int, [_dp, _dp, C.c_double, C.c_double, _dp]),
             "orc_camera_rays": (None, [C.POINTER(_abi.CameraDesc), C.c_uint64, C.c_int64, _dp, _dp]),
             "orc_transform_apply": (None, [_dp, C.c_int32, C.c_int64, _dp, _dp]),
+            "orc_prim_distance": (None, [C.c_void_p, C.c_int64, _dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), _dp, _dp]),
             "orc_light_shadow": (None, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), _dp, _dp, _dp,
                                         C.POINTER(C.c_int32)]),
             "orc_material_sample": (None, [C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_int64, _dp]),
@@ -240,6 +243,18 @@ class OracleScene:
         return dict(t=out[:, 0], kind=out[:, 1].astype(np.int32), inst=out[:, 2].astype(np.int32),
                     prim=out[:, 3].astype(np.int32), mat=out[:, 4].astype(np.int32), n=out[:, 5:8], p=out[:, 8:11],
                     counters=cnt.as_dict())
+
+    def prim_distance(self, rays, kind, inst, prim):
+        """FP64 distance along each ray to one given primitive (kind 1: triangle `prim` of
+        instance `inst`; kind 2: sphere `inst`), +inf where it is not hit in [tmin, tmax], and
+        the smallest barycentric coordinate of the crossing (1 for spheres)."""
+        rays = _f64(rays).reshape(-1, 8)
+        i32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+        kind, inst, prim = i32(kind), i32(inst), i32(prim)
+        out, margin = np.zeros(len(rays)), np.zeros(len(rays))
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        lib().orc_prim_distance(self.h, len(rays), _d(rays), ip(kind), ip(inst), ip(prim), _d(out), _d(margin))
+        return out, margin
 
     def env_total(self) -> float:
         return float(lib().orc_env_total(self.h))

@@ -170,10 +170,14 @@ struct Counters {
   uint64_t aabb = 0, tri = 0, sph = 0, inst = 0, rays_ext = 0, rays_shadow = 0, paths = 0, ext_bounces = 0;
   // the share of aabb / tri / sph / inst spent on extension (closest-hit) rays
   uint64_t aabb_ext = 0, tri_ext = 0, sph_ext = 0, inst_ext = 0;
+  // area-light samples with a non-zero contribution that reach the shadow test, and those of
+  // them whose shadow ray the sampled light's own surface blocks (other geometry ignored)
+  uint64_t light_samples = 0, light_self = 0;
   void add(const Counters& o) {
     aabb += o.aabb; tri += o.tri; sph += o.sph; inst += o.inst;
     rays_ext += o.rays_ext; rays_shadow += o.rays_shadow; paths += o.paths; ext_bounces += o.ext_bounces;
     aabb_ext += o.aabb_ext; tri_ext += o.tri_ext; sph_ext += o.sph_ext; inst_ext += o.inst_ext;
+    light_samples += o.light_samples; light_self += o.light_self;
   }
 };
 static thread_local Counters* tl_cnt = nullptr;
@@ -883,6 +887,7 @@ struct LightInfo {
 struct LightSample {
   V3 position, normal, emit;
   double pdf;
+  size_t idx;  // the light drawn (for the work counters)
 };
 
 struct Scene {
@@ -1014,10 +1019,25 @@ struct Scene {
     double pdf_omega = pdf_area * dist2 / cos_light;
     double pdf = pdf_select * pdf_omega;
     if (pdf <= 0.0) return false;
-    *out = LightSample{pos, nrm, info.emit, pdf};
+    *out = LightSample{pos, nrm, info.emit, pdf, idx};
     return true;
   }
 
+  // The part of World::hit that tests only light L's own primitive: its sphere, or its
+  // instance's mesh (world.rs:86-118), over [tmin, tmax).
+  bool light_blocks_itself(const LightInfo& L, const Ray& r, double tmin, double tmax) const {
+    Hit h;
+    if (L.kind == 0) return spheres[(size_t)L.idx].hit(r, tmin, tmax, &h);
+    const Instance& in = instances[(size_t)L.inst_id];
+    const Mesh& mesh = meshes[(size_t)in.mesh_id];
+    V3 o_obj = in.xform.apply_point_inv(r.o);
+    V3 d_obj = norm(in.xform.apply_vec_inv(r.d));
+    Ray r_obj{o_obj, d_obj, r.time};
+    int ti = -1;
+    if (!mesh.bvh.hit(mesh.tris, r_obj, tmin, 1e30, &h, &ti)) return false;
+    double t_world = dot(in.xform.apply_point(h.p) - r.o, r.d);
+    return t_world > tmin && t_world < tmax;
+  }
   bool hit(const Ray& r, double tmin, double tmax, Hit* best) const {  // world.rs:81-121
     double closest = tmax;
     bool found = false;
@@ -1098,7 +1118,22 @@ static V3 nee_area_light(const Scene& sc, const Material& mat, V3 beta, V3 hit_p
   Ray shadow{hit_p + RAY_EPSILON * wi, wi, ray.time};
   double tmax = fmx(dist - RAY_EPSILON, RAY_EPSILON);
   Hit tmp;
-  if (tl_cnt) tl_cnt->rays_shadow++;
+  if (tl_cnt) {
+    tl_cnt->rays_shadow++;
+    // counters only (no effect on the result): samples with a non-zero contribution, and the
+    // share of them that the light's own surface shadows
+    V3 f0;
+    double pb0;
+    mat_eval(mat, norm(-ray.d), wi, n, &f0, &pb0);
+    V3 c0 = had(had(beta, f0), ls.emit) * (c * mis_weight(ls.pdf, pb0) / ls.pdf);
+    if (ls.pdf > 0.0 && (c0.x != 0.0 || c0.y != 0.0 || c0.z != 0.0)) {
+      Counters* keep = tl_cnt;
+      tl_cnt = nullptr;  // the extra intersection test is not traversal work of the reference
+      keep->light_samples++;
+      if (sc.light_blocks_itself(sc.lights[ls.idx], shadow, RAY_EPSILON, tmax)) keep->light_self++;
+      tl_cnt = keep;
+    }
+  }
   if (sc.hit(shadow, RAY_EPSILON, tmax, &tmp)) return v3(0, 0, 0);
   V3 f;
   double pdf_bsdf;
@@ -1350,6 +1385,7 @@ extern "C" {
 typedef struct orc_counters {
   uint64_t aabb, tri, sph, inst, rays_ext, rays_shadow, paths, ext_bounces;
   uint64_t aabb_ext, tri_ext, sph_ext, inst_ext;  // part of the above spent on extension rays
+  uint64_t light_samples, light_self;             // area-light samples; those the light itself shadows
 } orc_counters;
 
 typedef struct orc_render_opts {
@@ -1436,6 +1472,47 @@ void orc_light_shadow(void* s, int64_t n, const int32_t* light, const double* p,
       out[i] = h.prim == L.tri_id ? 1 : 3;
     else
       out[i] = 2;
+  }
+}
+
+// ---- distance to one given primitive along each ray (World::hit restricted to it, world.rs:86-118):
+// kind 1 = triangle prim[i] (mesh-local) of instance inst[i], kind 2 = sphere inst[i]; rays[8i..]
+// = o, d, tmin, tmax; t_out[i] = the world distance, or +inf when the primitive is not hit;
+// margin[i] = for a triangle, the smallest barycentric coordinate of the ray's crossing of its
+// plane (negative outside the triangle), for a sphere 1
+void orc_prim_distance(void* s, int64_t n, const double* rays, const int32_t* kind, const int32_t* inst,
+                       const int32_t* prim, double* t_out, double* margin) {
+  const Scene& sc = *(Scene*)s;
+  for (int64_t i = 0; i < n; i++) {
+    const double* q = rays + 8 * i;
+    Ray r{v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]), 0.0};
+    const double tmin = q[6], tmax = q[7];
+    t_out[i] = INFINITY;
+    margin[i] = -INFINITY;
+    Hit h;
+    if (kind[i] == 2) {
+      margin[i] = 1.0;
+      if (inst[i] >= 0 && (size_t)inst[i] < sc.spheres.size() && sc.spheres[(size_t)inst[i]].hit(r, tmin, tmax, &h))
+        t_out[i] = h.t;
+    } else if (kind[i] == 1 && inst[i] >= 0 && (size_t)inst[i] < sc.instances.size()) {
+      const Instance& in = sc.instances[(size_t)inst[i]];
+      const Mesh& mesh = sc.meshes[(size_t)in.mesh_id];
+      if (prim[i] < 0 || (size_t)prim[i] >= mesh.tris.size()) continue;
+      Ray r_obj{in.xform.apply_point_inv(r.o), norm(in.xform.apply_vec_inv(r.d)), 0.0};
+      const Triangle& tr = mesh.tris[(size_t)prim[i]];
+      V3 v0, v1, v2;
+      tr.vertices_at(0.0, &v0, &v1, &v2);
+      V3 e1 = v1 - v0, e2 = v2 - v0, pv = cross(r_obj.d, e2);
+      double det = dot(e1, pv);
+      if (det != 0.0) {
+        V3 tv = r_obj.o - v0, qv = cross(tv, e1);
+        double u = dot(tv, pv) / det, v = dot(r_obj.d, qv) / det;
+        margin[i] = std::min(std::min(u, v), 1.0 - u - v);
+      }
+      if (!tr.hit(r_obj, tmin, 1e30, &h)) continue;
+      double t_world = dot(in.xform.apply_point(h.p) - r.o, r.d);
+      if (t_world > tmin && t_world < tmax) t_out[i] = t_world;
+    }
   }
 }
 
@@ -1588,7 +1665,7 @@ void orc_scene_trace(void* s, int64_t n, const double* rays, double* out, orc_co
     for (auto& c : cs) tot.add(c);
     // every query here is a closest-hit query
     *cnt = orc_counters{tot.aabb, tot.tri, tot.sph, tot.inst, tot.rays_ext, tot.rays_shadow, tot.paths, tot.ext_bounces,
-                        tot.aabb, tot.tri, tot.sph, tot.inst};
+                        tot.aabb, tot.tri, tot.sph, tot.inst, 0, 0};
   }
 }
 // env (env.rs): only for TPT_ENV_IMAGE scenes
@@ -1640,7 +1717,7 @@ double orc_render(void* s, const orc_render_opts* ro, double* acc, double* acc_w
   render(*sc, o, acc, acc_w, acc_sq, &tot);
   auto t1 = std::chrono::steady_clock::now();
   if (cnt) *cnt = orc_counters{tot.aabb, tot.tri, tot.sph, tot.inst, tot.rays_ext, tot.rays_shadow, tot.paths, tot.ext_bounces,
-                        tot.aabb_ext, tot.tri_ext, tot.sph_ext, tot.inst_ext};
+                               tot.aabb_ext, tot.tri_ext, tot.sph_ext, tot.inst_ext, tot.light_samples, tot.light_self};
   return std::chrono::duration<double>(t1 - t0).count();
 }
 // output.rs:96-102

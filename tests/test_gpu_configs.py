@@ -2,10 +2,8 @@
 
 Plain default scene (256x256, 64 spp, depth 8, rr 3, seed 1): the plain FP64 form and the FP32
 form against the oracle over the whole image.  Cornell box 512x512 (depth 12, rr 4): the FP32
-form against the oracle at 16 spp over the whole image.  Displaced-icosphere mesh (1.31M
-triangles, 2048x1024 environment, 1920x1080, depth 16) and the 4096-instance scene (4 x 1.31M
-triangles): the full frame rendered on the GPU, compared with the oracle on a centre crop (the
-oracle's per-ray loop over 4096 instances makes a full frame hours of CPU time).
+form against the oracle at 16 spp over the whole image.  The mesh and 4096-instance workloads are in
+test_gpu_heavy_scenes.py.
 """
 import numpy as np
 import pytest
@@ -49,30 +47,3 @@ def test_cornell_full_size(gpu_ctx):
                                                                seed=2))
     ref = oracle.OracleScene(sc).render(512, 512, 16, max_bounces=12, rr_start=4, seed=2, threads=oracle_threads())
     _report("cornell 512x512 fp32", *compare(out.acc, out.acc_sq, out.acc_w, ref["acc"], ref["acc_sq"], ref["acc_w"])[:4])
-
-
-def _crop_check(name, sc, w, h, spp, mb, rr, seed, crop, gpu_ctx):
-    out = T.render(T.DeviceScene(sc, gpu_ctx), T.RenderConfig(width=w, height=h, spp=spp, max_bounces=mb, rr_start=rr,
-                                                               seed=seed))
-    assert (out.acc_w == spp).all()
-    assert np.isfinite(out.acc).all()
-    x0, y0, x1, y1 = crop
-    ref = oracle.OracleScene(sc).render(w, h, spp, max_bounces=mb, rr_start=rr, seed=seed, threads=oracle_threads(),
-                                        crop=crop)
-    sl = (slice(y0, y1), slice(x0, x1))
-    _report(name, *compare(out.acc[sl], out.acc_sq[sl], out.acc_w[sl], ref["acc"][sl], ref["acc_sq"][sl],
-                           ref["acc_w"][sl])[:4])
-    return out
-
-
-def test_mesh_scene_full_size(gpu_ctx):
-    sc = scenes.mesh_scene(1920, 1080)
-    out = _crop_check("mesh 1920x1080 (64x64 centre crop)", sc, 1920, 1080, 32, 16, 3, 3, (928, 508, 992, 572), gpu_ctx)
-    print(f"mesh 1920x1080 32 spp: {out.stats}", flush=True)
-
-
-def test_instanced_scene_full_size(gpu_ctx):
-    sc = scenes.instanced_scene(1920, 1080)
-    out = _crop_check("instanced 1920x1080 (16x16 centre crop)", sc, 1920, 1080, 16, 8, 3, 3, (952, 532, 968, 548),
-                      gpu_ctx)
-    print(f"instanced 1920x1080 16 spp: {out.stats}", flush=True)

@@ -65,7 +65,7 @@ def load_library():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        if L.tpt_abi_version() != 1:
+        if L.tpt_abi_version() != _abi.ABI_VERSION:
             raise TptError(_abi.TPT_ERR_INTERNAL, "ABI version mismatch")
         _lib = L
     return _lib

@@ -96,6 +96,8 @@ class Stats(C.Structure):
         ("trace_kernel_ms", C.c_double),
         ("trace_launches", C.c_uint64),
         ("trace_rays", C.c_uint64),
+        ("light_samples", C.c_uint64),
+        ("light_self_occluded", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -162,5 +164,6 @@ ENTRY_POINTS = {
     "tpt_checkpoint_load": (C.c_int, [C.c_char_p, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_uint64), _dp, _dp,
                                       _ip]),
 }
+ABI_VERSION = 2  # TPT_ABI_VERSION of include/tinypt_gpu.h
 TONEMAP_NONE = 0
 TONEMAP_ACES = 1

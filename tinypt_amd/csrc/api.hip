@@ -611,9 +611,9 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   if (st) {
     std::vector<uint64_t> seg(TPT_STAT_WORDS);
     HIPC(hipMemcpy(seg.data(), W.ps.stats, TPT_STAT_WORDS * 8, hipMemcpyDeviceToHost));
-    uint64_t h[3] = {0, 0, 0};
+    uint64_t h[5] = {0, 0, 0, 0, 0};
     for (int g = 0; g < TPT_WF_SEGS; g++)
-      for (int k = 0; k < 3; k++) h[k] += seg[(size_t)g * TPT_STAT_STRIDE + k];
+      for (int k = 0; k < 5; k++) h[k] += seg[(size_t)g * TPT_STAT_STRIDE + k];
     st->paths = h[2];
     st->rays_extension = h[0];
     st->rays_shadow = h[1];
@@ -621,6 +621,8 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
     st->trace_kernel_ms = tt.pairs_ms();
     st->trace_launches = launches;
     st->trace_rays = h[0];
+    st->light_samples = h[3];
+    st->light_self_occluded = h[4];
   }
   (void)traced;
   return TPT_OK;
@@ -690,6 +692,8 @@ int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
     st->trace_kernel_ms = st->elapsed_ms;
     st->trace_launches = (uint64_t)((cfg->spp + chunk - 1) / chunk);
     st->trace_rays = h[0] + h[1];
+    st->light_samples = 0;  // the plain form traces every shadow ray in full (no separate decision)
+    st->light_self_occluded = 0;
   }
   return TPT_OK;
 }

@@ -464,6 +464,8 @@ int adjoint_render(hipStream_t s, const SceneF32& S, const SceneF64& S64, const 
     stats->trace_kernel_ms = ms;
     stats->trace_launches = 1;
     stats->trace_rays = h[0] + h[1];
+    stats->light_samples = 0;
+    stats->light_self_occluded = 0;
   }
   return TPT_OK;
 }

@@ -327,6 +327,7 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W, int 
   qm.load(W.ps.counters, TPT_CNT_SHL, W.ps.seg_cap);
   if constexpr (!GEN) qp.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
   const uint32_t n = qm.n;
+  uint32_t dropped = 0;  // self-occluded samples this thread saw
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     uint32_t slot = 0;
@@ -354,7 +355,12 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W, int 
     }
     const uint32_t k = seg_slot(keep, W.ps, TPT_CNT_SHK, s_slot);  // every thread of the block
     if (keep && k != UINT32_MAX) W.ps.shl_keep[k] = slot;
+    if (j < n && !keep) dropped++;
   }
+  for (int off = 32; off > 0; off >>= 1) dropped += (uint32_t)__shfl_down((int)dropped, off);
+  uint64_t* st = W.ps.stats + (blockIdx.x & (TPT_WF_SEGS - 1)) * TPT_STAT_STRIDE;
+  if (__lane_id() == 0 && dropped) atomicAdd((unsigned long long*)&st[4], (unsigned long long)dropped);
+  if (gtid() == 0 && n) atomicAdd((unsigned long long*)&W.ps.stats[3], (unsigned long long)n);
 }
 
 // Any hit for one shadow queue (0: environment, 1: area light); an unoccluded ray adds its

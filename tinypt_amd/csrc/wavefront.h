@@ -71,7 +71,8 @@ struct PathState {
                     // 53-bit uniforms are drawn again from it for the self-occlusion test)
   uint32_t* shl_keep;  // segmented queue of light-queue slots whose ray is traced (not self-occluded)
   uint32_t* counters;
-  uint64_t* stats;  // per segment g: [g * TPT_STAT_STRIDE + k], k = 0 extension rays, 1 shadow rays, 2 paths
+  uint64_t* stats;  // per segment g: [g * TPT_STAT_STRIDE + k], k = 0 extension rays, 1 shadow rays, 2 paths,
+                    // 3 area-light samples, 4 of them shadowed by their own light
   uint32_t seg_cap; // entries per queue segment (wf_seg_cap of the allocated path count)
   uint32_t pad;
 };
