@@ -9,9 +9,12 @@ full render of that image (67.1M camera paths).
 Printed (one JSON line):  paths/s of the FP32 wavefront render (value), Mrays/s, the plain
 FP64 GPU form's paths/s on the same workload, RMSE of both against a 2048-spp FP64 plain
 render, the roofline of the dominant kernel (extension traversal) and the FP64 CPU oracle
-timed on a bounded sample on this host.
+timed on a bounded sample on this host.  Under "workloads", the other configurations, each
+timed once or a few times: the default scene at 256x256, 64 spp (FP32 and plain FP64), the
+1.31M-triangle mesh at 1920x1080, 1024 spp, and the 4096-instance scene at 1920x1080, adaptive
+64..4096 spp, threshold 0.01.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-heavy]
 """
 import argparse
 import json
@@ -37,13 +40,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_per_ray(counters):
-    """Bytes one extension (closest-hit) World::hit query reads and writes, from the reference
-    traversal's work counts on the same scene: 24 per AABB slab test (FP32 box), 36 per
+def reference_bytes_per_ray(counters):
+    """Bytes the reference's traversal would read and write per extension (closest-hit) query on
+    the same scene, from the oracle's work counts: 24 per AABB slab test (FP32 box), 36 per
     triangle test, 16 per sphere test, 48 per instance visit (FP32 affine inverse), 32 ray in +
-    16 hit out."""
+    16 hit out.  A diagnostic of the work the four-wide trees and the caches absorb, not the
+    kernel's traffic."""
     c = counters
     return (24.0 * c["aabb_ext"] + 36.0 * c["tri_ext"] + 16.0 * c["sph_ext"] + 48.0 * c["inst_ext"]) / c["rays_ext"] + 48.0
+
+
+# Compulsory HBM bytes of one extension ray: the first bounce generates its camera rays and
+# writes a 12-byte hit record; later bounces read the 32-byte ray (origin + pdf, direction +
+# flags) and write the hit.  The scene's nodes and triangles are read through the caches.
+EXTEND_BYTES_FIRST = 12.0
+EXTEND_BYTES_LATER = 44.0
 
 
 def cpu_baseline(scene, seconds_target=15.0):
@@ -77,6 +88,7 @@ def cpu_model():
 
 
 TRAFFIC_PROFILE = "profiles/wavefront_cornell_512x512_256spp_hbm_counters.md"
+VALU_PROFILE = "profiles/wavefront_cornell_512x512_256spp_valu_l2_counters.md"
 
 
 def measured_traffic():
@@ -89,6 +101,20 @@ def measured_traffic():
     for line in open(path):
         if line.startswith("k_extend_traffic_bytes_per_launch:"):
             return float(line.split(":")[1])
+    return None
+
+
+def measured_valu():
+    """VALU busy and lane utilisation (percent) of the later-bounce extension kernel, from the
+    counter table in VALU_PROFILE."""
+    path = os.path.join(ROOT, VALU_PROFILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| `void tpt::k_extend<false, false>"):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            return {"valu_busy_pct": float(cols[3]), "valu_lane_utilisation_pct": float(cols[4]),
+                    "occupancy_pct": float(cols[5]), "source": VALU_PROFILE}
     return None
 
 
@@ -111,6 +137,71 @@ def hip_copy_calibration(nbytes):
         hip.hipFree(b)
 
 
+def _timed(ds, cfg, reps):
+    """Mean wall time of `reps` renders into device buffers after one warm-up render, with the
+    stats of the last."""
+    import tinypt_amd as T
+    import torch
+    dev = torch.device("cuda", ds.ctx.device)
+    acc = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float64, device=dev)
+    acc_w = torch.empty((cfg.height, cfg.width), dtype=torch.float64, device=dev)
+    T.render_device(ds, cfg, acc, acc_w)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = T.render_device(ds, cfg, acc, acc_w)
+    secs = (time.perf_counter() - t0) / reps
+    return secs, st, acc_w
+
+
+def other_workloads(ctx, heavy=True):
+    """The other configurations: the default scene at 256x256, 64 spp, depth 8, rr 3, seed 1 in
+    both precisions; with `heavy`, the 1.31M-triangle mesh (1920x1080, 1024 spp, depth 16, rr 3)
+    and the 4096-instance scene (1920x1080, adaptive 64..4096 spp, threshold 0.01, depth 8)."""
+    import tinypt_amd as T
+    from tinypt_amd import scenes
+    out = {}
+    ds = T.DeviceScene(scenes.default_scene(256, 256, "black"), ctx)
+    kw = dict(width=256, height=256, spp=64, max_bounces=8, rr_start=3, seed=1)
+    s32, st32, _ = _timed(ds, T.RenderConfig(precision=T.FP32_WAVEFRONT, **kw), 20)
+    s64, _, _ = _timed(ds, T.RenderConfig(precision=T.FP64_PLAIN, **kw), 5)
+    paths = 256 * 256 * 64
+    out["default_256x256_64spp"] = {"fp32_ms": s32 * 1e3, "fp32_paths_per_s": paths / s32,
+                                    "fp32_mrays_per_s": (st32["rays_extension"] + st32["rays_shadow"]) / s32 / 1e6,
+                                    "fp64_plain_ms": s64 * 1e3, "fp64_plain_paths_per_s": paths / s64,
+                                    "speedup_fp32_vs_fp64_plain": s64 / s32}
+    log(f"default 256x256 64 spp: fp32 {s32 * 1e3:.3f} ms, plain fp64 {s64 * 1e3:.3f} ms ({s64 / s32:.1f}x)")
+    ds.close()
+    if not heavy:
+        return out
+    t0 = time.perf_counter()
+    ds = T.DeviceScene(scenes.mesh_scene(1920, 1080), ctx)
+    build = time.perf_counter() - t0
+    cfg = T.RenderConfig(width=1920, height=1080, spp=1024, max_bounces=16, rr_start=3, seed=3)
+    secs, st, _ = _timed(ds, cfg, 1)
+    paths = 1920 * 1080 * 1024
+    out["mesh_1920x1080_1024spp"] = {"ms": secs * 1e3, "paths": paths, "paths_per_s": paths / secs,
+                                     "mrays_per_s": (st["rays_extension"] + st["rays_shadow"]) / secs / 1e6,
+                                     "extension_kernel_ms": st["trace_kernel_ms"], "scene_build_s": build,
+                                     "triangles": 1310722}
+    log(f"mesh 1080p 1024 spp: {secs:.3f} s, {paths / secs / 1e6:.1f} M paths/s")
+    ds.close()
+    t0 = time.perf_counter()
+    ds = T.DeviceScene(scenes.instanced_scene(1920, 1080), ctx)
+    build = time.perf_counter() - t0
+    cfg = T.RenderConfig(width=1920, height=1080, spp=4096, max_bounces=8, rr_start=3, seed=3, adaptive_enabled=True,
+                         adaptive_min_spp=64, adaptive_threshold=0.01)
+    secs, st, acc_w = _timed(ds, cfg, 1)
+    w = acc_w.cpu().numpy()
+    out["instanced_4096_1920x1080_adaptive_64_4096"] = {
+        "ms": secs * 1e3, "paths": st["paths"], "paths_per_s": st["paths"] / secs,
+        "mrays_per_s": (st["rays_extension"] + st["rays_shadow"]) / secs / 1e6, "mean_spp": float(w.mean()),
+        "min_spp": float(w.min()), "max_spp": float(w.max()), "extension_kernel_ms": st["trace_kernel_ms"],
+        "scene_build_s": build}
+    log(f"instanced 4096 adaptive: {secs:.2f} s, {st['paths'] / secs / 1e6:.1f} M paths/s, mean spp {w.mean():.0f}")
+    ds.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +212,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU render")
     ap.add_argument("--no-inverse", action="store_true", help="skip the inverse-rendering gradient loop")
     ap.add_argument("--inverse-steps", type=int, default=200)
+    ap.add_argument("--no-heavy", action="store_true", help="skip the mesh and 4096-instance workloads")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -199,12 +291,12 @@ def main():
     bytes_per_ray = None
     if not args.no_cpu:
         cpu, counters = cpu_baseline(scene)
-        bytes_per_ray = algorithmic_bytes_per_ray(counters)
+        bytes_per_ray = reference_bytes_per_ray(counters)
         log(f"cpu oracle: {cpu}")
     else:
         import oracle
         r = oracle.OracleScene(scene).render(64, 64, 16, max_bounces=MAX_BOUNCES, rr_start=RR_START, seed=SEED)
-        bytes_per_ray = algorithmic_bytes_per_ray(r["counters"])
+        bytes_per_ray = reference_bytes_per_ray(r["counters"])
 
     inverse = None
     if not args.no_inverse:
@@ -224,16 +316,26 @@ def main():
                    "config": "cornell + 16 GGX spheres, 256x256, 16 spp, depth 8, target 1024 spp, Adam lr 0.02"}
         log(f"inverse: {inverse}")
 
+    workloads = other_workloads(ds.ctx, heavy=not args.no_heavy)
+
+    # roofline of the extension kernel: its compulsory bytes per launch over its measured launch
+    # time (HIP events around every launch, on the render's stream)
     avg_launch_ms = trace_ms / max(trace_launches, 1)
     rays_per_launch = trace_rays / max(trace_launches, 1)
-    achieved = bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    first_rays = float(paths)  # one camera ray per path on the first bounce
+    bytes_per_render = EXTEND_BYTES_FIRST * first_rays + EXTEND_BYTES_LATER * max(trace_rays - first_rays, 0.0)
+    bytes_per_launch = bytes_per_render / max(trace_launches, 1)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     traffic = measured_traffic()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": traffic, "kernel": "k_extend (FP32 closest-hit traversal)",
-                "algorithmic_bytes_per_ray": bytes_per_ray, "avg_launch_ms": avg_launch_ms,
-                "rays_per_launch": rays_per_launch,
+                "algorithmic_bytes": "per ray: 12 B hit written on the first bounce (camera rays generated), "
+                                     "32 B ray read + 12 B hit written on later bounces; nodes and triangles cached",
+                "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_launch_ms, "rays_per_launch": rays_per_launch,
                 "traffic_source": TRAFFIC_PROFILE if traffic else None,
-                "hbm_gbs_from_traffic": traffic / (avg_launch_ms * 1e-3) / 1e9 if traffic else None}
+                "hbm_gbs_from_traffic": traffic / (avg_launch_ms * 1e-3) / 1e9 if traffic else None,
+                "limiter": {"kind": "VALU issue (divergent per-ray traversal work)", **(measured_valu() or {})},
+                "reference_traversal_bytes_per_ray": bytes_per_ray}
 
     line = {"metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": secs * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
@@ -245,7 +347,7 @@ def main():
             "host_buffers_paths_per_s": paths / host_secs, "fp64_plain_paths_per_s": plain_rate,
             "speedup_vs_fp64_plain": value / plain_rate if plain_rate else None,
             "rmse": rmse, "grad_s_per_iter": inverse["s_per_iter"] if inverse else None, "inverse": inverse,
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "workloads": workloads}
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
     print(json.dumps(line), flush=True)

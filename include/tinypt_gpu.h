@@ -176,7 +176,8 @@ typedef struct tpt_scene_info {
   int32_t max_blas_depth;
   int32_t max_tlas_depth;
   int32_t n_lights;
-  int32_t reserved;
+  int32_t traversal_stack;  /* deepest traversal stack of the FP32 kernels (entries per ray); the
+                               kernels keep 24 per lane in LDS and the rest in a global area */
   int64_t device_bytes;
 } tpt_scene_info;
 int tpt_scene_get_info(const tpt_scene* scene, tpt_scene_info* info);
@@ -245,6 +246,17 @@ int tpt_render_plan(const tpt_render_config* cfg, int64_t npix, uint32_t path_bu
  * the mesh's original order, -1 for spheres).  Host pointers. */
 int tpt_trace_closest(tpt_ctx* ctx, const tpt_scene* scene, int64_t n, const float* rays,
                       float* hit_t, int32_t* hit_kind, int32_t* hit_inst, int32_t* hit_prim);
+
+/* tpt_trace_closest for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid allocation): rays and outputs are device
+ * pointers of the context's device, laid out as for tpt_trace_closest, and the call allocates
+ * nothing -- the traversal's overflow stacks live in a fixed 16 MiB scratch per device, held
+ * until the process ends (its grid is capped so that they fit).  Fails with
+ * TPT_ERR_UNSUPPORTED for a scene whose traversal stack is too deep for that scratch.  Calls
+ * that overlap on one device race on the scratch: issue them from one host thread at a time.
+ * Synchronous.  No counterpart in the reference. */
+int tpt_trace_closest_noalloc(tpt_ctx* ctx, const tpt_scene* scene, int64_t n, const float* d_rays, float* d_hit_t,
+                              int32_t* d_hit_kind, int32_t* d_hit_inst, int32_t* d_hit_prim);
 
 /* Differentiable parameters of the adjoint: per material, d/d(param[0..3]) as laid out in
  * mat_param (albedo rgb + alpha for GGX, albedo rgb for Lambert, emit rgb for DiffuseLight). */

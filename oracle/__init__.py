@@ -69,6 +69,9 @@ def lib():
             "orc_aabb_hit": (C.c_int, [_dp, _dp, C.c_double, C.c_double, _dp]),
             "orc_camera_rays": (None, [C.POINTER(_abi.CameraDesc), C.c_uint64, C.c_int64, _dp, _dp]),
             "orc_transform_apply": (None, [_dp, C.c_int32, C.c_int64, _dp, _dp]),
+            "orc_task_tiles": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
+            "orc_checkpoint_save": (C.c_int32, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, _dp, _dp]),
+            "orc_checkpoint_load": (C.c_int32, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32, _u64p, _dp, _dp]),
             "orc_prim_distance": (None, [C.c_void_p, C.c_int64, _dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), _dp, _dp]),
             "orc_light_shadow": (None, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), _dp, _dp, _dp,
@@ -300,6 +303,34 @@ class OracleScene:
         cnt = Counters()
         secs = lib().orc_render(self.h, C.byref(o), _d(acc), _d(acc_w), _d(acc_sq), C.byref(cnt))
         return dict(acc=acc, acc_w=acc_w, acc_sq=acc_sq, seconds=float(secs), counters=cnt.as_dict())
+
+
+def task_tiles(w, h, tile=16, morton=True) -> np.ndarray:
+    """The tile tasks of a w x h image in task-id order (render.rs:76-106): (n, 4) x0, y0, x1, y1."""
+    n = int(lib().orc_task_tiles(w, h, tile, 1 if morton else 0, None))
+    out = np.zeros((n, 4), dtype=np.int64)
+    lib().orc_task_tiles(w, h, tile, 1 if morton else 0, out.ctypes.data_as(C.POINTER(C.c_int64)))
+    return out
+
+
+def checkpoint_save(path, scene_hash, next_id, acc, acc_w):
+    """save_checkpoint (checkpoint.rs:48-80) of acc (h, w, 3) and acc_w (h, w)."""
+    h, w = acc_w.shape
+    rc = lib().orc_checkpoint_save(str(path).encode(), scene_hash & 0xFFFFFFFFFFFFFFFF, w, h, next_id,
+                                   _d(_f64(acc)), _d(_f64(acc_w)))
+    if rc != 0:
+        raise OSError(f"checkpoint write failed: {path}")
+
+
+def checkpoint_load(path, scene_hash, w, h):
+    """load_checkpoint (checkpoint.rs:84-138): (next_id, acc, acc_w), or None."""
+    acc, acc_w = np.zeros((h, w, 3)), np.zeros((h, w))
+    nid = C.c_uint64()
+    rc = lib().orc_checkpoint_load(str(path).encode(), scene_hash & 0xFFFFFFFFFFFFFFFF, w, h, C.byref(nid), _d(acc),
+                                   _d(acc_w))
+    if rc < 0:
+        raise OSError(f"checkpoint read failed: {path}")
+    return (int(nid.value), acc, acc_w) if rc == 1 else None
 
 
 def resolve_pixels(acc, acc_w) -> np.ndarray:

@@ -20,6 +20,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -1215,6 +1216,20 @@ static uint64_t morton2(size_t tx, size_t ty, size_t tiles_x, size_t tiles_y) { 
 struct Task {
   size_t x0, y0, x1, y1;
 };
+// The tile tasks of a w x h image in task-id order (render.rs:76-106): row-major tiles, then a
+// stable sort by the Morton code of the tile position when enabled.
+static std::vector<Task> make_tasks(size_t w, size_t h, size_t tile, bool morton) {
+  std::vector<Task> tasks;
+  for (size_t y = 0; y < h; y += tile)
+    for (size_t x = 0; x < w; x += tile) tasks.push_back(Task{x, y, std::min(x + tile, w), std::min(y + tile, h)});
+  if (morton) {
+    size_t tiles_x = (w + tile - 1) / tile, tiles_y = (h + tile - 1) / tile;
+    std::stable_sort(tasks.begin(), tasks.end(), [&](const Task& a, const Task& b) {
+      return morton2(a.x0 / tile, a.y0 / tile, tiles_x, tiles_y) < morton2(b.x0 / tile, b.y0 / tile, tiles_x, tiles_y);
+    });
+  }
+  return tasks;
+}
 
 struct RenderOpts {
   int width, height, spp, max_bounces, rr_start, tile;
@@ -1282,16 +1297,7 @@ static void render(const Scene& sc, const RenderOpts& o, double* acc, double* ac
   // does not change any value; results are written straight to the output buffers.
   size_t w = (size_t)o.width, h = (size_t)o.height;
   double inv_w = 1.0 / (double)w, inv_h = 1.0 / (double)h;
-  size_t tile = (size_t)std::max(o.tile, 1);
-  std::vector<Task> tasks;
-  for (size_t y = 0; y < h; y += tile)
-    for (size_t x = 0; x < w; x += tile) tasks.push_back(Task{x, y, std::min(x + tile, w), std::min(y + tile, h)});
-  if (o.morton) {
-    size_t tiles_x = (w + tile - 1) / tile, tiles_y = (h + tile - 1) / tile;
-    std::stable_sort(tasks.begin(), tasks.end(), [&](const Task& a, const Task& b) {
-      return morton2(a.x0 / tile, a.y0 / tile, tiles_x, tiles_y) < morton2(b.x0 / tile, b.y0 / tile, tiles_x, tiles_y);
-    });
-  }
+  std::vector<Task> tasks = make_tasks(w, h, (size_t)std::max(o.tile, 1), o.morton);
   int nthreads = o.threads > 0 ? o.threads : (int)std::max(1u, std::thread::hardware_concurrency());
   std::atomic<size_t> next{0};
   std::vector<Counters> cnts((size_t)nthreads);
@@ -1514,6 +1520,62 @@ void orc_prim_distance(void* s, int64_t n, const double* rays, const int32_t* ki
       if (t_world > tmin && t_world < tmax) t_out[i] = t_world;
     }
   }
+}
+
+// ---- tile tasks (render.rs:76-106): n tasks, out[4i..] = x0, y0, x1, y1 in task-id order
+int64_t orc_task_tiles(int32_t w, int32_t h, int32_t tile, int32_t morton, int64_t* out) {
+  std::vector<Task> t = make_tasks((size_t)w, (size_t)h, (size_t)std::max(tile, 1), morton != 0);
+  if (out)
+    for (size_t i = 0; i < t.size(); i++) {
+      out[4 * i] = (int64_t)t[i].x0; out[4 * i + 1] = (int64_t)t[i].y0;
+      out[4 * i + 2] = (int64_t)t[i].x1; out[4 * i + 3] = (int64_t)t[i].y1;
+    }
+  return (int64_t)t.size();
+}
+
+// ---- checkpoint files (checkpoint.rs:48-138): "HYPCKPT\0", u32 version 1, u64 scene hash,
+// u32 w, u32 h, u64 next task id, w*h*3 f64 acc, w*h f64 acc_w; little endian (this host is);
+// written to path.tmp and renamed
+int32_t orc_checkpoint_save(const char* path, uint64_t hash, uint32_t w, uint32_t h, uint64_t next_id,
+                            const double* acc, const double* acc_w) {
+  std::string tmp = std::string(path) + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return -1;
+  const uint32_t ver = 1;
+  bool ok = std::fwrite("HYPCKPT\0", 1, 8, f) == 8 && std::fwrite(&ver, 4, 1, f) == 1 && std::fwrite(&hash, 8, 1, f) == 1 &&
+            std::fwrite(&w, 4, 1, f) == 1 && std::fwrite(&h, 4, 1, f) == 1 && std::fwrite(&next_id, 8, 1, f) == 1;
+  const size_t n = (size_t)w * h;
+  ok = ok && std::fwrite(acc, 8, 3 * n, f) == 3 * n && std::fwrite(acc_w, 8, n, f) == n;
+  ok = std::fclose(f) == 0 && ok;
+  if (!ok) return -1;
+  return std::rename(tmp.c_str(), path) == 0 ? 0 : -1;
+}
+// returns 1 and fills the outputs when the file exists with this magic, version, hash and size;
+// 0 when it does not (load_checkpoint's None); -1 on a read error
+int32_t orc_checkpoint_load(const char* path, uint64_t hash, uint32_t w, uint32_t h, uint64_t* next_id, double* acc,
+                            double* acc_w) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return 0;
+  char magic[8];
+  uint32_t ver = 0, fw = 0, fh = 0;
+  uint64_t fhash = 0, nid = 0;
+  int rc = 0;
+  if (std::fread(magic, 1, 8, f) != 8) rc = -1;
+  else if (std::memcmp(magic, "HYPCKPT\0", 8) != 0) rc = 0;
+  else if (std::fread(&ver, 4, 1, f) != 1) rc = -1;
+  else if (ver != 1) rc = 0;
+  else if (std::fread(&fhash, 8, 1, f) != 1) rc = -1;
+  else if (fhash != hash) rc = 0;
+  else if (std::fread(&fw, 4, 1, f) != 1 || std::fread(&fh, 4, 1, f) != 1) rc = -1;
+  else if (fw != w || fh != h) rc = 0;
+  else if (std::fread(&nid, 8, 1, f) != 1) rc = -1;
+  else {
+    const size_t n = (size_t)w * h;
+    rc = std::fread(acc, 8, 3 * n, f) == 3 * n && std::fread(acc_w, 8, n, f) == n ? 1 : -1;
+    *next_id = nid;
+  }
+  std::fclose(f);
+  return rc;
 }
 
 // ---- transform (transform.rs:24-93).  xf[12] = A row-major, t; op 0 apply_point, 1 apply_point_inv,

@@ -120,12 +120,12 @@ class SceneInfo(C.Structure):
         ("max_blas_depth", C.c_int32),
         ("max_tlas_depth", C.c_int32),
         ("n_lights", C.c_int32),
-        ("reserved", C.c_int32),
+        ("traversal_stack", C.c_int32),
         ("device_bytes", C.c_int64),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 # Every entry point include/tinypt_gpu.h declares: name -> (restype, argtypes)
@@ -144,6 +144,7 @@ ENTRY_POINTS = {
     "tpt_render": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), _dp, _dp, _dp, C.POINTER(Stats)]),
     "tpt_render_device": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), _vp, _vp, _vp, C.POINTER(Stats)]),
     "tpt_trace_closest": (C.c_int, [_vp, _vp, C.c_int64, _fp, _fp, _ip, _ip, _ip]),
+    "tpt_trace_closest_noalloc": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]),
     "tpt_render_plan": (C.c_int, [C.POINTER(RenderConfigC), C.c_int64, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.POINTER(PassPlan)]),
     "tpt_render_adjoint": (C.c_int, [_vp, _vp, C.POINTER(RenderConfigC), _fp, _dp, C.POINTER(Stats)]),

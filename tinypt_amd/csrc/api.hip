@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,6 +104,18 @@ struct tpt_scene {
   size_t device_bytes = 0;
 };
 
+// Fixed scratch of tpt_trace_closest_noalloc: the traversal kernel's per-thread overflow stacks
+// (the entries beyond the 24 per lane it keeps in LDS), for callers that cannot allocate.  One
+// copy per device, held until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address is looked up once per device in
+// tpt_ctx_create (hipGetSymbolAddress, with that device current) and only that address is
+// passed to kernels.
+#define TPT_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define TPT_MAX_DEVICES 256
+static __device__ __align__(256) int32_t scratch[TPT_NOALLOC_SCRATCH_BYTES / 4];
+static int32_t* g_scratch_dev[TPT_MAX_DEVICES];  // device address of `scratch` per device
+static std::mutex g_scratch_mu;
+
 static int upload_materials(tpt_scene* sc) {
   hipStream_t s = sc->ctx->stream;
   const HostScene& H = sc->host;
@@ -149,7 +162,17 @@ int tpt_ctx_create(int device, tpt_ctx** out) {
   int n = 0;
   HIPC(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(TPT_ERR_INVALID_ARGUMENT, "no such HIP device");
+  if (device >= TPT_MAX_DEVICES) return fail(TPT_ERR_UNSUPPORTED, "device index above the library's device table");
   HIPC(hipSetDevice(device));
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+      void* p = nullptr;
+      HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) return fail(TPT_ERR_DEVICE, "no device address for the fixed scratch");
+      g_scratch_dev[device] = static_cast<int32_t*>(p);
+    }
+  }
   std::unique_ptr<tpt_ctx> c(new tpt_ctx());
   c->device = device;
   HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -335,7 +358,7 @@ int tpt_scene_get_info(const tpt_scene* sc, tpt_scene_info* info) {
   info->max_blas_depth = H.max_blas_depth;
   info->max_tlas_depth = H.max_tlas_depth;
   info->n_lights = (int32_t)H.lights.size();
-  info->reserved = 0;
+  info->traversal_stack = H.stack4;
   info->device_bytes = (int64_t)sc->device_bytes;
   return TPT_OK;
 }
@@ -816,6 +839,34 @@ int tpt_trace_closest(tpt_ctx* ctx, const tpt_scene* sc, int64_t n, const float*
   HIPC(hipMemcpyAsync(hit_kind, k.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   HIPC(hipMemcpyAsync(hit_inst, i.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   HIPC(hipMemcpyAsync(hit_prim, p.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  return TPT_OK;
+}
+
+int tpt_trace_closest_noalloc(tpt_ctx* ctx, const tpt_scene* sc, int64_t n, const float* d_rays, float* d_hit_t,
+                              int32_t* d_hit_kind, int32_t* d_hit_inst, int32_t* d_hit_prim) {
+  if (!ctx || !sc || n < 0 || (n > 0 && (!d_rays || !d_hit_t || !d_hit_kind || !d_hit_inst || !d_hit_prim)))
+    return fail(TPT_ERR_INVALID_ARGUMENT, "null argument");
+  if (sc->ctx != ctx) return fail(TPT_ERR_INVALID_ARGUMENT, "scene was not uploaded with this context");
+  if (n == 0) return TPT_OK;
+  int32_t* scr = g_scratch_dev[ctx->device];
+  if (!scr) return fail(TPT_ERR_INTERNAL, "the fixed scratch of this device was not set up");
+  const int gcap = needed_gstack(sc->host);
+  uint32_t max_blocks = 0;  // default grid when the stack fits in LDS (no scratch used)
+  if (gcap > 0) {
+    // a grid whose overflow stacks fit the scratch; the kernel loops over the rays
+    const uint64_t fit = (uint64_t)(TPT_NOALLOC_SCRATCH_BYTES / 4) / (uint64_t)gcap / trace_block_threads();
+    if (fit == 0)
+      return fail(TPT_ERR_UNSUPPORTED, "the scene's traversal stack (" + std::to_string(sc->host.stack4) +
+                                           " entries) needs more overflow space per block than the " +
+                                           std::to_string(TPT_NOALLOC_SCRATCH_BYTES >> 20) +
+                                           " MiB fixed scratch holds; use tpt_trace_closest");
+    max_blocks = (uint32_t)std::min<uint64_t>(fit, trace_grid_threads(n) / trace_block_threads());
+  }
+  HIPC(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  HIPC(launch_trace_closest(sc->s32, n, d_rays, d_hit_t, d_hit_kind, d_hit_inst, d_hit_prim, gcap > 0 ? scr : nullptr,
+                            gcap, s, max_blocks));
   HIPC(hipStreamSynchronize(s));
   return TPT_OK;
 }

@@ -41,10 +41,13 @@ static inline uint32_t tr_grid(int64_t n) {
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 uint32_t trace_grid_threads(int64_t n) { return tr_grid(n) * TPT_TR_BLOCK; }
+uint32_t trace_block_threads() { return TPT_TR_BLOCK; }
 
 hipError_t launch_trace_closest(const SceneF32& S, int64_t n, const float* rays, float* t, int32_t* kind, int32_t* inst,
-                                int32_t* prim, int32_t* gstack, int32_t gcap, hipStream_t s) {
-  hipLaunchKernelGGL(k_trace_closest, dim3(tr_grid(n)), dim3(TPT_TR_BLOCK), 0, s, S, n, (const float4*)rays, t, kind, inst,
+                                int32_t* prim, int32_t* gstack, int32_t gcap, hipStream_t s, uint32_t max_blocks) {
+  uint32_t blocks = tr_grid(n);
+  if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks;  // the kernel loops over its rays
+  hipLaunchKernelGGL(k_trace_closest, dim3(blocks), dim3(TPT_TR_BLOCK), 0, s, S, n, (const float4*)rays, t, kind, inst,
                      prim, gstack, gcap);
   return hipGetLastError();
 }

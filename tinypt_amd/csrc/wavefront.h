@@ -166,8 +166,11 @@ hipError_t wf_launch_accumulate_adaptive(const WfParams& W, const AdaptiveArgs& 
 uint32_t wf_grid_threads(uint32_t n);
 
 // trace_closest entry (k_trace.hip)
+// max_blocks > 0 caps the grid (each thread then traces several rays); gstack holds gcap
+// overflow entries per thread of the launched grid
 hipError_t launch_trace_closest(const SceneF32& S, int64_t n, const float* rays, float* t, int32_t* kind, int32_t* inst,
-                                int32_t* prim, int32_t* gstack, int32_t gcap, hipStream_t s);
-uint32_t trace_grid_threads(int64_t n);
+                                int32_t* prim, int32_t* gstack, int32_t gcap, hipStream_t s, uint32_t max_blocks = 0);
+uint32_t trace_grid_threads(int64_t n);  // threads of the default grid for n rays
+uint32_t trace_block_threads();          // threads per block
 
 }  // namespace tpt
