@@ -18,26 +18,37 @@ ARCH = os.environ.get("TPT_OFFLOAD_ARCH", "gfx950")
 
 
 def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    scene_obj = os.path.join(HERE, "scene_host.o")
-    hook_obj = os.path.join(HERE, "host_trace.o")
-    cmds = [
-        ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC, "-c",
-         os.path.join(CSRC, "scene_host.cpp"), "-o", scene_obj],
-        [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-function", "-Xclang",
-         "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-c", os.path.join(HERE, "host_trace.hip"),
-         "-o", hook_obj],
-        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", hook_obj, scene_obj, "-o", tmp],
-    ]
-    for c in cmds:
-        if verbose:
-            print(" ".join(c), flush=True)
-        r = subprocess.run(c, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("host_trace build failed:\n" + " ".join(c) + "\n" + r.stdout)
-    os.replace(tmp, LIB)
-    os.remove(scene_obj)
-    os.remove(hook_obj)
+    """Builds the library; concurrent callers (pytest workers) take turns on a lock file and the
+    later ones find it current."""
+    import fcntl
+    with open(os.path.join(HERE, ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        if os.path.exists(LIB) and not is_stale():
+            return LIB
+        tag = f"{os.getpid()}"
+        tmp = f"{LIB}.tmp{tag}"
+        scene_obj = os.path.join(HERE, f"scene_host.{tag}.o")
+        hook_obj = os.path.join(HERE, f"host_trace.{tag}.o")
+        cmds = [
+            ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC, "-c",
+             os.path.join(CSRC, "scene_host.cpp"), "-o", scene_obj],
+            [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-function", "-Xclang",
+             "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-c", os.path.join(HERE, "host_trace.hip"),
+             "-o", hook_obj],
+            [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", hook_obj, scene_obj, "-o", tmp],
+        ]
+        try:
+            for c in cmds:
+                if verbose:
+                    print(" ".join(c), flush=True)
+                r = subprocess.run(c, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+                if r.returncode != 0:
+                    raise RuntimeError("host_trace build failed:\n" + " ".join(c) + "\n" + r.stdout)
+            os.replace(tmp, LIB)
+        finally:
+            for f in (scene_obj, hook_obj, tmp):
+                if os.path.exists(f):
+                    os.remove(f)
     return LIB
 
 
@@ -52,4 +63,6 @@ def is_stale() -> bool:
 
 
 if __name__ == "__main__":
+    if os.path.exists(LIB):
+        os.utime(os.path.join(HERE, "host_trace.hip"))  # an explicit build always compiles
     print(build(verbose="-v" in sys.argv))
