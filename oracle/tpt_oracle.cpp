@@ -14,7 +14,14 @@
 // seeding mode.
 //
 // Files followed: rng.rs, math.rs, ray.rs, geometry.rs, bvh.rs, transform.rs, world.rs,
-// material.rs, env.rs, integrator.rs, render.rs:42-299, output.rs:96-102, constants.rs.
+// material.rs, env.rs, integrator.rs, render.rs:42-299, output.rs:96-102, constants.rs,
+// checkpoint.rs:48-138.
+//
+// How each part is pinned without the reference binary (tests/test_oracle_*.py,
+// tests/test_loader.py, tests/test_output.py): closed forms, exact rational arithmetic, brute
+// force, the reference's own unit tests restated, and independent restatements of the integer
+// specifications.  Every function exported below has such a check; parity unpinned: only the
+// bit equality of the whole with the Rust binary, which no build here can run.
 
 #include <algorithm>
 #include <atomic>

@@ -28,7 +28,7 @@ SCENES = os.path.join(os.path.dirname(__file__), "golden", "scenes")
 
 
 @pytest.mark.parametrize("name,w,h,spp", [("cornell.xml", 48, 48, 256), ("default.xml", 64, 36, 128),
-                                          ("mesh.xml", 48, 36, 128)])
+                                          ("mesh.xml", 64, 36, 128)])
 def test_loaded_sample_scene_matches_oracle(gpu_ctx, name, w, h, spp):
     cfg = T.RenderConfig()
     warns = []
