@@ -6,7 +6,9 @@ sampled direction or a branch.  Its central difference with common random number
 is then the exact derivative of the same realisation up to O(h^2), and the adjoint must match
 it within 1%.  GGX roughness moves the sampled directions; the adjoint takes its gradient as a
 per-path central difference with the same seeds, which sums to the render's own central
-difference, so it is held to the same 1% (with Russian roulette on as well).
+difference, so it is held to the same 1% (with Russian roulette on as well).  Agreement in
+expectation with the finite differences of the plain FP64 render and of the oracle, roughness
+included, is tested on the inverse-rendering problem in test_gpu_inverse_gradients.py.
 """
 import numpy as np
 import pytest
@@ -103,27 +105,6 @@ def test_adjoint_on_two_level_scene(gpu_ctx):
     assert (np.abs(ad - fd)[sig] / np.abs(fd)[sig]).max() < 0.01
 
 
-def test_roughness_gradient_in_expectation(gpu_ctx):
-    """Over independent seeds the roughness gradient agrees with the render's central difference
-    in expectation as well (the per-seed agreement is checked above)."""
-    sc = material_zoo()
-    w = h = 16
-    ds, base, dl = _setup(gpu_ctx, sc, w, h)
-    ggx = [i for i, m in enumerate(sc.mats) if m.kind == _abi.MAT_GGX]
-    ad, fd = [], []
-    for seed in range(4):
-        cfg = T.RenderConfig(width=w, height=h, spp=64, max_bounces=6, rr_start=3, seed=100 + seed)
-        g, _ = T.render_adjoint(ds, cfg, dl)
-        ad.append([g[i, 3] for i in ggx])
-        cfg2 = T.RenderConfig(width=w, height=h, spp=64, max_bounces=6, rr_start=3, seed=500 + seed)
-        fd.append([_fd(ds, cfg2, dl, base, i, 3) for i in ggx])
-    ad, fd = np.array(ad), np.array(fd)
-    se = np.sqrt(ad.var(0, ddof=1) / len(ad) + fd.var(0, ddof=1) / len(fd))
-    z = (ad.mean(0) - fd.mean(0)) / (se + 1e-30)
-    print(f"roughness: adjoint {ad.mean(0)} fd {fd.mean(0)} z {z}", flush=True)
-    assert np.all(np.abs(z) < 4.0), z
-
-
 def test_adjoint_accumulates_into_grad(gpu_ctx):
     sc = scenes.cornell_box(16, 16)
     ds, base, dl = _setup(gpu_ctx, sc, 16, 16)
@@ -195,6 +176,6 @@ def test_adjoint_against_fp64_plain_and_oracle_fd(gpu_ctx):
     assert np.median(rel) < 1e-2
     assert np.all(np.abs(z(fd64, fdref)) < 4.0)
     assert np.all(np.abs(z(ad, fdref)) < 4.0), z(ad, fdref)
-    # and the means agree to a few percent of the gradient's scale
-    scale = np.abs(fdref.mean(0)).max()
-    assert np.all(np.abs(ad.mean(0) - fdref.mean(0)) < 0.05 * scale + 4 * np.sqrt(ad.var(0) / n + fdref.var(0) / n))
+    # the comparison resolves each parameter: its standard error is a small part of its gradient
+    se = np.sqrt(ad.var(0, ddof=1) / n + fdref.var(0, ddof=1) / n)
+    assert np.all(se < 0.1 * np.abs(fdref.mean(0))), se / np.abs(fdref.mean(0))
