@@ -137,15 +137,16 @@ def hip_copy_calibration(nbytes):
         hip.hipFree(b)
 
 
-def _timed(ds, cfg, reps):
-    """Mean wall time of `reps` renders into device buffers after one warm-up render, with the
-    stats of the last."""
+def _timed(ds, cfg, reps, warm=True):
+    """Mean wall time of `reps` renders into device buffers (after one warm-up render when
+    `warm`), with the stats of the last."""
     import tinypt_amd as T
     import torch
     dev = torch.device("cuda", ds.ctx.device)
     acc = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float64, device=dev)
     acc_w = torch.empty((cfg.height, cfg.width), dtype=torch.float64, device=dev)
-    T.render_device(ds, cfg, acc, acc_w)
+    if warm:
+        T.render_device(ds, cfg, acc, acc_w)
     t0 = time.perf_counter()
     for _ in range(reps):
         st = T.render_device(ds, cfg, acc, acc_w)
@@ -153,13 +154,16 @@ def _timed(ds, cfg, reps):
     return secs, st, acc_w
 
 
-def other_workloads(ctx, heavy=True):
+def other_workloads(ctx, heavy=True, only=None):
     """The other configurations: the default scene at 256x256, 64 spp, depth 8, rr 3, seed 1 in
     both precisions; with `heavy`, the 1.31M-triangle mesh (1920x1080, 1024 spp, depth 16, rr 3)
-    and the 4096-instance scene (1920x1080, adaptive 64..4096 spp, threshold 0.01, depth 8)."""
+    and the 4096-instance scene (1920x1080, adaptive 64..4096 spp, threshold 0.01, depth 8).
+    `only` names one of them ("default", "mesh", "instanced") to run it alone."""
     import tinypt_amd as T
     from tinypt_amd import scenes
     out = {}
+    if only not in (None, "default"):
+        return _heavy_workloads(ctx, out, only)
     ds = T.DeviceScene(scenes.default_scene(256, 256, "black"), ctx)
     kw = dict(width=256, height=256, spp=64, max_bounces=8, rr_start=3, seed=1)
     s32, st32, _ = _timed(ds, T.RenderConfig(precision=T.FP32_WAVEFRONT, **kw), 20)
@@ -171,13 +175,21 @@ def other_workloads(ctx, heavy=True):
                                     "speedup_fp32_vs_fp64_plain": s64 / s32}
     log(f"default 256x256 64 spp: fp32 {s32 * 1e3:.3f} ms, plain fp64 {s64 * 1e3:.3f} ms ({s64 / s32:.1f}x)")
     ds.close()
-    if not heavy:
+    if not heavy or only == "default":
         return out
+    return _heavy_workloads(ctx, out, None)
+
+
+def _heavy_workloads(ctx, out, only):
+    import tinypt_amd as T
+    from tinypt_amd import scenes
+    if only == "instanced":
+        return _instanced_workload(ctx, out)
     t0 = time.perf_counter()
     ds = T.DeviceScene(scenes.mesh_scene(1920, 1080), ctx)
     build = time.perf_counter() - t0
     cfg = T.RenderConfig(width=1920, height=1080, spp=1024, max_bounces=16, rr_start=3, seed=3)
-    secs, st, _ = _timed(ds, cfg, 1)
+    secs, st, _ = _timed(ds, cfg, 1, warm=False)
     paths = 1920 * 1080 * 1024
     out["mesh_1920x1080_1024spp"] = {"ms": secs * 1e3, "paths": paths, "paths_per_s": paths / secs,
                                      "mrays_per_s": (st["rays_extension"] + st["rays_shadow"]) / secs / 1e6,
@@ -185,12 +197,20 @@ def other_workloads(ctx, heavy=True):
                                      "triangles": 1310722}
     log(f"mesh 1080p 1024 spp: {secs:.3f} s, {paths / secs / 1e6:.1f} M paths/s")
     ds.close()
+    if only == "mesh":
+        return out
+    return _instanced_workload(ctx, out)
+
+
+def _instanced_workload(ctx, out):
+    import tinypt_amd as T
+    from tinypt_amd import scenes
     t0 = time.perf_counter()
     ds = T.DeviceScene(scenes.instanced_scene(1920, 1080), ctx)
     build = time.perf_counter() - t0
     cfg = T.RenderConfig(width=1920, height=1080, spp=4096, max_bounces=8, rr_start=3, seed=3, adaptive_enabled=True,
                          adaptive_min_spp=64, adaptive_threshold=0.01)
-    secs, st, acc_w = _timed(ds, cfg, 1)
+    secs, st, acc_w = _timed(ds, cfg, 1, warm=False)
     w = acc_w.cpu().numpy()
     out["instanced_4096_1920x1080_adaptive_64_4096"] = {
         "ms": secs * 1e3, "paths": st["paths"], "paths_per_s": st["paths"] / secs,
@@ -213,6 +233,9 @@ def main():
     ap.add_argument("--no-inverse", action="store_true", help="skip the inverse-rendering gradient loop")
     ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--no-heavy", action="store_true", help="skip the mesh and 4096-instance workloads")
+    ap.add_argument("--no-workloads", action="store_true", help="skip every workload but the Cornell box")
+    ap.add_argument("--only", choices=["default", "mesh", "instanced"],
+                    help="time one of the other workloads alone and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -223,6 +246,10 @@ def main():
 
     import tinypt_amd as T
     from tinypt_amd import scenes
+
+    if args.only:
+        print(json.dumps({"workloads": other_workloads(T.default_context(), only=args.only)}), flush=True)
+        return
 
     scene = scenes.cornell_box(W, H)
     ds = T.DeviceScene(scene)
@@ -316,7 +343,7 @@ def main():
                    "config": "cornell + 16 GGX spheres, 256x256, 16 spp, depth 8, target 1024 spp, Adam lr 0.02"}
         log(f"inverse: {inverse}")
 
-    workloads = other_workloads(ds.ctx, heavy=not args.no_heavy)
+    workloads = None if args.no_workloads else other_workloads(ds.ctx, heavy=not args.no_heavy)
 
     # roofline of the extension kernel: its compulsory bytes per launch over its measured launch
     # time (HIP events around every launch, on the render's stream)

@@ -63,6 +63,18 @@ struct DevBuf {
   T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// HIP events kept by a context between calls: creating and destroying an event per timed
+// launch cost tens of microseconds per small render.
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  EventPool() = default;
+  EventPool(const EventPool&) = delete;
+  EventPool& operator=(const EventPool&) = delete;
+  ~EventPool() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);  // teardown: nothing to report to
+  }
+};
+
 template <class T>
 hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
   hipError_t e = b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
@@ -85,6 +97,7 @@ struct tpt_ctx {
   // device sums of renders into host buffers (grow-only)
   DevBuf out_acc, out_w, out_sq;
   size_t ps_cap = 0;
+  EventPool ev_total, ev_trace;  // timing events of the render calls (one call at a time)
 };
 
 struct tpt_scene {
@@ -494,27 +507,29 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   return W;
 }
 
+// Records events on a stream from a pool (growing it as needed); pairs_ms sums the intervals
+// between marks 2i and 2i + 1.  A pool serves one timer at a time.
 struct EventTimer {
   hipStream_t s;
   bool on;
-  std::vector<hipEvent_t> ev;
-  EventTimer(hipStream_t st, bool enable) : s(st), on(enable) {}
-  ~EventTimer() {
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  }
+  EventPool* pool;
+  size_t used = 0;
+  EventTimer(hipStream_t st, bool enable, EventPool* p) : s(st), on(enable), pool(p) {}
   hipError_t mark() {
     if (!on) return hipSuccess;
-    hipEvent_t e;
-    hipError_t r = hipEventCreate(&e);
-    if (r != hipSuccess) return r;
-    ev.push_back(e);
-    return hipEventRecord(e, s);
+    if (used == pool->ev.size()) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      pool->ev.push_back(e);
+    }
+    return hipEventRecord(pool->ev[used++], s);
   }
-  double pairs_ms() {  // sum of (ev[2i+1] - ev[2i])
+  double pairs_ms() {
     double t = 0;
-    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    for (size_t i = 0; i + 1 < used; i += 2) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) t += ms;
+      if (hipEventElapsedTime(&ms, pool->ev[i], pool->ev[i + 1]) == hipSuccess) t += ms;
     }
     return t;
   }
@@ -570,7 +585,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   WfParams W = make_wf(c, sc, cfg, gcap);
   HIPC(hipMemsetAsync(W.ps.stats, 0, TPT_STAT_WORDS * 8, s));
   HIPC(hipMemsetAsync(W.ps.counters, 0, TPT_NUM_CNT * 4, s));
-  EventTimer total(s, st != nullptr), tt(s, st != nullptr);
+  EventTimer total(s, st != nullptr, &c->ev_total), tt(s, st != nullptr, &c->ev_trace);
   uint64_t launches = 0, traced = 0;
   HIPC(total.mark());
   if (!cfg->adaptive_enabled) {
@@ -690,7 +705,7 @@ int render_plain(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   P.welford = c->welford.as<double>();
   P.done = c->done.as<uint8_t>();
   P.stats = c->stats.as<uint64_t>();
-  EventTimer total(s, st != nullptr);
+  EventTimer total(s, st != nullptr, &c->ev_total);
   HIPC(total.mark());
   const int chunk = 8;
   if (cfg->spp == 0) {
