@@ -34,6 +34,8 @@ extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const flo
   S.tri_orig = H.tri_orig.data();
   S.tri_inst = H.flat ? H.tri_inst32.data() : nullptr;
   S.inst = H.inst32.data();
+  S.inst64 = H.inst64.data();
+  S.inst_c64 = H.inst_c64.data();
   S.tlas_items = H.tlas_items.data();
   S.sph = H.sph.data();
   S.sph_mat = H.sph_mat.data();
@@ -98,6 +100,8 @@ extern "C" int tpt_test_selfocc(const tpt_scene_desc* d, int64_t n, const float*
   S.tri_orig = H.tri_orig.data();
   S.tri_inst = H.flat ? H.tri_inst32.data() : nullptr;
   S.inst = H.inst32.data();
+  S.inst64 = H.inst64.data();
+  S.inst_c64 = H.inst_c64.data();
   S.tlas_items = H.tlas_items.data();
   S.sph = H.sph.data();
   S.sph_mat = H.sph_mat.data();

@@ -103,7 +103,7 @@ struct tpt_ctx {
 struct tpt_scene {
   tpt_ctx* ctx = nullptr;  // null for a host-only scene (tpt_scene_build_host)
   HostScene host;
-  DevBuf nodes4, tris32, tri_mat32, tri_orig, tri_inst, inst32, tlas_items, sph64, sph_mat;
+  DevBuf nodes4, tris32, tri_mat32, tri_orig, tri_inst, inst32, inst_c64, tlas_items, sph64, sph_mat;
   DevBuf ref_nodes, ref_indices, tri_v, tri_mat, meshes, inst64;
   DevBuf mat_kind, mat_p32, mat_p64;
   DevBuf l32, cdf32, l64, cdf64;
@@ -222,6 +222,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   HIPC(upload(sc->tri_orig, H.tri_orig, s));
   if (H.flat) HIPC(upload(sc->tri_inst, H.tri_inst32, s));
   HIPC(upload(sc->inst32, H.inst32, s));
+  HIPC(upload(sc->inst_c64, H.inst_c64, s));
   HIPC(upload(sc->tlas_items, H.tlas_items, s));
   HIPC(upload(sc->sph64, sph64, s));
   HIPC(upload(sc->sph_mat, H.sph_mat, s));
@@ -270,6 +271,8 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   S.tri_inst = H.flat ? sc->tri_inst.as<int32_t>() : nullptr;
   S.flat_inst = H.flat ? H.flat_inst : -1;
   S.inst = sc->inst32.as<InstF32>();
+  S.inst64 = sc->inst64.as<InstF64>();
+  S.inst_c64 = sc->inst_c64.as<double>();
   S.tlas_items = sc->tlas_items.as<int32_t>();
   S.sph = sc->sph64.as<double>();
   S.sph_mat = sc->sph_mat.as<int32_t>();
@@ -301,7 +304,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   C.lens_radius = (float)H.cam.lens_radius;
   C.focus_dist = (float)H.cam.focus_dist;
   HIPC(hipStreamSynchronize(s));
-  for (DevBuf* b : {&sc->nodes4, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->tri_inst, &sc->inst32, &sc->tlas_items, &sc->sph64,
+  for (DevBuf* b : {&sc->nodes4, &sc->tris32, &sc->tri_mat32, &sc->tri_orig, &sc->tri_inst, &sc->inst32, &sc->inst_c64, &sc->tlas_items, &sc->sph64,
                     &sc->sph_mat, &sc->ref_nodes, &sc->ref_indices, &sc->tri_v, &sc->tri_mat, &sc->meshes, &sc->inst64,
                     &sc->mat_kind, &sc->mat_p32, &sc->mat_p64, &sc->l32, &sc->cdf32, &sc->l64, &sc->cdf64, &sc->env_rgb32,
                     &sc->alias_prob, &sc->alias_idx, &sc->cell_pdf, &sc->env_rgb64, &sc->row_cdf, &sc->col_cdf})

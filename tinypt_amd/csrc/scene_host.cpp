@@ -626,6 +626,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
   // ---- instances
   inst64.resize((size_t)ni);
   inst32.resize((size_t)ni);
+  inst_c64.assign(3 * (size_t)ni, 0.0);
   std::vector<DAabb> item_box;
   std::vector<int32_t> item_code;
   for (int i = 0; i < ni; i++) {
@@ -656,6 +657,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
     y.pad = 0;
     if (!valid || blas[(size_t)x.mesh].nodes.empty()) continue;
     const DAabb wb = instance_world_box(mesh_cut[(size_t)x.mesh], x.a, x.t);
+    for (int k = 0; k < 3; k++) inst_c64[3 * (size_t)i + k] = 0.5 * (wb.mn[k] + wb.mx[k]);
     item_box.push_back(wb);
     item_code.push_back(i);
   }
@@ -722,6 +724,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
       y.mat_override = -1;
       y.mesh = -1;
       flat_inst = (int32_t)inst32.size();
+      inst_c64.insert(inst_c64.end(), {0.0, 0.0, 0.0});  // the flattened tree is traversed in world space
       inst32.push_back(y);
       item_box.clear();
       item_code.clear();

@@ -105,6 +105,8 @@ struct HostScene {
   bool flat = false;                // instances flattened into one world-space tree
   int32_t flat_inst = -1;           // index of the identity pseudo-instance that holds that tree
   std::vector<InstF32> inst32;
+  std::vector<double> inst_c64;   // 3 per inst32 entry: the world point an entering ray is re-based
+                                  // to (the centre of the instance's world box), FP64
   std::vector<float> sph32;  // 4 per sphere (FP32 copy, TLAS boxes only)
   std::vector<LightF32> lights32;
   std::vector<float> light_cdf32;

@@ -6,6 +6,11 @@
 //     so object-space t equals world t and the running closest distance culls every level;
 //     the reference's object-space bound t_obj >= tmin on the normalised ray becomes
 //     t >= tmin / |A^-1 d|, combined with the world bound t > tmin;
+//   * an entering ray is first re-based, in FP64, to its point nearest the instance's centre
+//     (t shifted by s, the instance's distances taken relative to it): an FP32 object-space
+//     origin far from the object (a camera 50 units away, an instance scaled by 0.2) would
+//     carry absolute errors of 1e-5 -- a few thousandths of a triangle of a 1.3M-triangle mesh
+//     -- and let rays through the cracks between neighbouring triangles;
 //   * Moller-Trumbore in FP32, two-sided, |det| < 1e-10 * |A^-1 d| rejected (the reference's
 //     1e-10 applies to the normalised object ray);
 //   * spheres use the Ray Tracing Gems (Haines et al. 2019, ch. 7) quadratic evaluated in FP64
@@ -33,6 +38,8 @@ struct SceneF32 {
   const int32_t* __restrict__ tri_orig;
   const int32_t* __restrict__ tri_inst;  // flattened slots: original instance | TPT_TRI_FLIP
   const InstF32* __restrict__ inst;
+  const InstF64* __restrict__ inst64;    // A^-1 and t in FP64, to re-base entering rays
+  const double* __restrict__ inst_c64;   // per instance: the world point rays are re-based to
   const int32_t* __restrict__ tlas_items;
   const double* __restrict__ sph;  // c.xyz, r (FP64)
   const int32_t* __restrict__ sph_mat;
@@ -119,6 +126,7 @@ struct TraceF32 {
   vec3<float> co, cd, inv, oi;    // ray of the current level (object space inside an instance)
   uint32_t nx_w, ny_w, nz_w, nx, ny, nz;
   float tmin;
+  float shift;    // world t of the current level's origin (0 at the top level)
   float lo;       // lower t bound at the current level
   float det_eps;
   float best_t;
@@ -147,6 +155,7 @@ struct TraceF32 {
     nx_w = near_off(inv_w.x, 0); ny_w = near_off(inv_w.y, 1); nz_w = near_off(inv_w.z, 2);
     nx = nx_w; ny = ny_w; nz = nz_w;
     lo = tmin;
+    shift = 0.f;
     det_eps = 1e-10f;
     best_t = rtmax;
     best_key = -1;
@@ -226,9 +235,12 @@ struct TraceF32 {
     }
     if (have) {
       if (ref == TPT_STACK_SENTINEL) {
-        // ---- leaving an instance: back to the world ray
+        // ---- leaving an instance: back to the world ray (best_t back in world units: it is the
+        // best hit's distance, or the ray's tmax while nothing is hit)
         co = o; cd = d; inv = inv_w; oi = oi_w; lo = tmin; det_eps = 1e-10f; cur_inst = -1;
         nx = nx_w; ny = ny_w; nz = nz_w;
+        best_t = hit.t;
+        shift = 0.f;
       } else if (cur_inst < 0) {
         // ---- top-level leaf: one item
         const int32_t item = S.tlas_items[(uint32_t)ref & TPT_LEAF_START_MASK];
@@ -257,13 +269,24 @@ struct TraceF32 {
           if (root != TPT_REF_EMPTY) {
             const float4* w = reinterpret_cast<const float4*>(I.w2o);
             const float4 r0 = w[0], r1 = w[1], r2 = w[2];
-            co = mk(fmaf(r0.x, o.x, fmaf(r0.y, o.y, fmaf(r0.z, o.z, r0.w))),
-                    fmaf(r1.x, o.x, fmaf(r1.y, o.y, fmaf(r1.z, o.z, r1.w))),
-                    fmaf(r2.x, o.x, fmaf(r2.y, o.y, fmaf(r2.z, o.z, r2.w))));
+            // re-base the origin in FP64 to o + s d, the ray's point nearest the instance's
+            // centre c (s rounded to FP32 first, so that the shift of t is exactly s)
+            {
+              const double* c = S.inst_c64 + 3 * (size_t)item;
+              const InstF64& J = S.inst64[item];
+              const double dx = d.x, dy = d.y, dz = d.z, ox = o.x, oy = o.y, oz = o.z;
+              shift = (float)(((c[0] - ox) * dx + (c[1] - oy) * dy + (c[2] - oz) * dz) / (dx * dx + dy * dy + dz * dz));
+              const double s = shift;
+              const double px = ox + s * dx - J.t[0], py = oy + s * dy - J.t[1], pz = oz + s * dz - J.t[2];
+              const double* m = J.a_inv;
+              co = mk((float)(m[0] * px + m[1] * py + m[2] * pz), (float)(m[3] * px + m[4] * py + m[5] * pz),
+                      (float)(m[6] * px + m[7] * py + m[8] * pz));
+            }
             cd = mk(fmaf(r0.x, d.x, fmaf(r0.y, d.y, r0.z * d.z)), fmaf(r1.x, d.x, fmaf(r1.y, d.y, r1.z * d.z)),
                     fmaf(r2.x, d.x, fmaf(r2.y, d.y, r2.z * d.z)));
             const float len = sqrtf(dot(cd, cd));
-            lo = fmaxf(tmin, tmin / len);
+            lo = fmaxf(tmin, tmin / len) - shift;
+            best_t -= shift;
             det_eps = 1e-10f * len;
             inv = mk(safe_rcp(cd.x), safe_rcp(cd.y), safe_rcp(cd.z));
             oi = mk(co.x * inv.x, co.y * inv.y, co.z * inv.z);
@@ -304,7 +327,7 @@ struct TraceF32 {
           }
           if (th < best_t || key >= best_key) {
             best_t = th; best_key = key; found = true;
-            hit.t = th; hit.inst = hinst; hit.prim = (int32_t)(start + k);
+            hit.t = th + shift; hit.inst = hinst; hit.prim = (int32_t)(start + k);
             if (ANY) return false;
           }
         }
