@@ -121,7 +121,10 @@ typedef struct tpt_render_config {
   uint64_t seed;                /* RenderConfig.seed                                          */
   int32_t samples_per_pass;     /* wavefront batching: samples of every pixel in flight per
                                    pass (0 = library default); no effect on results           */
-  int32_t reserved;
+  int32_t traversal_refill;     /* FP32 wavefront traversal: 0 = chosen by the scene's shape;
+                                   -1 = one ray per lane; n > 0 = persistent waves that take
+                                   new rays once fewer than min(n, 63) lanes still trace (no
+                                   effect on results)                                         */
 } tpt_render_config;
 
 /* ---- counters ------------------------------------------------------------------------------ */

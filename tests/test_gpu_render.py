@@ -165,6 +165,20 @@ def test_samples_per_pass_does_not_change_the_image(gpu_ctx):
         assert o.stats["rays_extension"] == outs[0].stats["rays_extension"]
 
 
+@pytest.mark.parametrize("name", ["cornell", "material_zoo", "mesh_small", "instanced_small"])
+def test_traversal_refill_does_not_change_the_image(gpu_ctx, name):
+    """The persistent refilling traversal kernels find the same hits as the one-ray-per-lane
+    kernels: every threshold, and the scene-shape default, give bit-identical sums."""
+    build, w, h, spp, mb, rr, seed = PARITY_CASES[name]
+    ds = T.DeviceScene(build(), gpu_ctx)
+    kw = dict(width=w, height=h, spp=spp // 4, max_bounces=mb, rr_start=rr, seed=seed)
+    outs = [T.render(ds, T.RenderConfig(traversal_refill=r, **kw)) for r in (-1, 0, 1, 16, 48, 63, 200)]
+    for o in outs[1:]:
+        assert np.array_equal(o.acc, outs[0].acc) and np.array_equal(o.acc_sq, outs[0].acc_sq)
+        for k in ("rays_extension", "rays_shadow", "light_samples", "light_self_occluded"):
+            assert o.stats[k] == outs[0].stats[k], k
+
+
 def test_seed_changes_the_image(gpu_ctx):
     sc = scenes.cornell_box(16, 16)
     ds = T.DeviceScene(sc, gpu_ctx)

@@ -83,7 +83,7 @@ class RenderConfigC(C.Structure):
         ("adaptive_threshold", C.c_double),
         ("seed", C.c_uint64),
         ("samples_per_pass", C.c_int32),
-        ("reserved", C.c_int32),
+        ("traversal_refill", C.c_int32),
     ]
 
 

@@ -112,6 +112,7 @@ k_extend(WfParams W, int cur, PassArgs A) {
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, gtid())] = 0;
     W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHK, gtid())] = 0;
   }
+  if (gtid() < 2) W.ps.counters[TPT_CNT_FETCH_SH(gtid())] = 0;  // cursors of this bounce's shadow kernels
   if (gtid() == 0) atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
   for (uint32_t i = gtid(); i < n; i += gridDim.x * blockDim.x) {
     f3 o, d;
@@ -126,6 +127,78 @@ k_extend(WfParams W, int cur, PassArgs A) {
     HitF32 h;
     trace_f32<false>(W.S, o, d, (float)TPT_RAY_EPSILON, 1e30f, &h, st);
     W.ps.hit[i] = HitRec{__float_as_int(h.t), h.inst, h.prim};
+  }
+}
+
+// Persistent form of k_extend for deep trees, where the rays of one wave need very different
+// numbers of traversal steps: a wave keeps its lanes' traversals resumable (TraceF32 step by
+// step) and, once fewer than W.refill lanes are still tracing, hands its idle lanes the next rays
+// of the queue (one atomic per refill on a global cursor; dynamic ray fetch, Aila and Laine
+// 2009).  The hits are those of k_extend: each ray's traversal is the same sequence of steps.
+// The cursor (TPT_CNT_FETCH) is zero at launch: the render clears it, and every k_shade launch
+// clears it for the next extension launch.  Every wave exits once the cursor has passed the
+// queue's end and its lanes are idle.
+template <bool OVF, bool GEN>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(GEN ? 5 : 6)))
+k_extend_refill(WfParams W, int cur, PassArgs A) {
+  __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
+  QueueMap qm;
+  uint32_t n;
+  if constexpr (GEN) {
+    n = A.n_paths;
+  } else {
+    qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
+    n = qm.n;
+  }
+  const float4* __restrict__ po = W.ps.o[cur];
+  const float4* __restrict__ pd = W.ps.d[cur];
+  LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
+                     TPT_WF_STACK};
+  if (gtid() < TPT_WF_SEGS) {  // clear the queues the shading kernel fills next
+    W.ps.counters[TPT_CNT_IDX(cur ? TPT_CNT_Q0 : TPT_CNT_Q1, gtid())] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SH, gtid())] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHL, gtid())] = 0;
+    W.ps.counters[TPT_CNT_IDX(TPT_CNT_SHK, gtid())] = 0;
+  }
+  if (gtid() < 2) W.ps.counters[TPT_CNT_FETCH_SH(gtid())] = 0;  // cursors of this bounce's shadow kernels
+  if (gtid() == 0) atomicAdd((unsigned long long*)&W.ps.stats[0], (unsigned long long)n);
+  uint32_t* cursor = W.ps.counters + TPT_CNT_FETCH;
+  const uint32_t lane = __lane_id();
+  TraceF32<false> tr;
+  bool live = false, exhausted = false;
+  uint32_t my = 0;
+  const int threshold = min(W.refill, 64);  // nl < threshold <= 64: every refill moves the cursor
+  for (;;) {
+    const uint64_t mask = __ballot(live);
+    const int nl = __popcll(mask);
+    if (!exhausted && nl < threshold) {  // wave-uniform: refill the idle lanes
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(cursor, (uint32_t)(64 - nl));
+      base = (uint32_t)__shfl((int)base, 0);
+      if (base >= n) exhausted = true;
+      if (!live) {
+        my = base + (uint32_t)__popcll(~mask & ((1ull << lane) - 1ull));
+        if (my < n) {
+          f3 o, d;
+          if constexpr (GEN) {
+            RngF32 rng(1);
+            gen_path(W, A, my, &rng, &o, &d);
+          } else {
+            const uint32_t slot = qm.slot(my);
+            o = f3of(po[slot]);
+            d = f3of(pd[slot]);
+          }
+          live = tr.init(W.S, o, d, (float)TPT_RAY_EPSILON, 1e30f);
+          if (!live) W.ps.hit[my] = HitRec{__float_as_int(tr.hit.t), tr.hit.inst, tr.hit.prim};
+        }
+      }
+      continue;
+    }
+    if (nl == 0) break;  // the queue is taken and every lane is idle
+    if (live && !tr.step(W.S, st)) {
+      W.ps.hit[my] = HitRec{__float_as_int(tr.hit.t), tr.hit.inst, tr.hit.prim};
+      live = false;
+    }
   }
 }
 
@@ -152,6 +225,7 @@ k_shade(WfParams W, int cur, PassArgs A) {
   }
   const int nxt = cur ^ 1;
   const uint32_t stride = gridDim.x * blockDim.x;
+  if (gtid() == 0) W.ps.counters[TPT_CNT_FETCH] = 0;  // for the next extension launch (k_extend_refill)
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     bool alive = false, has_env = false, has_light = false;
@@ -472,6 +546,68 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_accumulate_adaptive(WfParams W
               (unsigned long long)kept);
 }
 
+// Persistent form of k_shadow for deep trees (see k_extend_refill): lanes whose shadow ray is
+// done take the queue's next rays once fewer than W.refill_sh lanes are still tracing.  The
+// cursor TPT_CNT_FETCH_SH(queue) is cleared by the extension kernel of the same bounce.
+template <bool OVF>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+k_shadow_refill(WfParams W, int queue) {
+  __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
+  QueueMap qm;
+  qm.load(W.ps.counters, queue ? TPT_CNT_SHK : TPT_CNT_SH, W.ps.seg_cap);
+  const uint32_t n = qm.n;
+  const float4* __restrict__ qo = queue ? W.ps.sh_l : W.ps.sh_o[0];
+  const float4* __restrict__ qd = W.ps.sh_d[queue];
+  const float4* __restrict__ qc = W.ps.sh_c[queue];
+  LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
+                     TPT_WF_STACK};
+  uint32_t* cursor = W.ps.counters + TPT_CNT_FETCH_SH(queue);
+  const uint32_t lane = __lane_id();
+  TraceF32<true> tr;
+  bool live = false, exhausted = false;
+  uint32_t slot = 0, traced = 0;
+  int32_t id = 0;
+  const int threshold = min(W.refill_sh, 64);  // nl < threshold <= 64: every refill moves the cursor
+  for (;;) {
+    const uint64_t mask = __ballot(live);
+    const int nl = __popcll(mask);
+    if (!exhausted && nl < threshold) {  // wave-uniform: refill the idle lanes
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(cursor, (uint32_t)(64 - nl));
+      base = (uint32_t)__shfl((int)base, 0);
+      if (base >= n) exhausted = true;
+      if (!live) {
+        const uint32_t j = base + (uint32_t)__popcll(~mask & ((1ull << lane) - 1ull));
+        if (j < n) {
+          slot = queue ? W.ps.shl_keep[qm.slot(j)] : qm.slot(j);
+          const float4 o4 = qo[slot], d4 = qd[slot];
+          f3 o = f3of(o4);
+          float tmax = o4.w;
+          if (queue) {  // light queue: o4 is the shading point, sh_c's w the distance to the light sample
+            o = o + (float)TPT_RAY_EPSILON * f3of(d4);
+            tmax = light_shadow_tmax32(qc[slot].w);
+          }
+          id = __float_as_int(d4.w);
+          traced++;
+          live = tr.init(W.S, o, f3of(d4), (float)TPT_RAY_EPSILON, tmax);
+          if (!live && !tr.found) add_radiance(W.ps.L, id, f3of(qc[slot]));
+        }
+      }
+      continue;
+    }
+    if (nl == 0) break;  // the queue is taken and every lane is idle
+    if (live && !tr.step(W.S, st)) {
+      if (!tr.found) add_radiance(W.ps.L, id, f3of(qc[slot]));  // unoccluded
+      live = false;
+    }
+  }
+  uint32_t tot = traced;
+  for (int off = 32; off > 0; off >>= 1) tot += (uint32_t)__shfl_down((int)tot, off);
+  if (__lane_id() == 0 && tot)
+    atomicAdd((unsigned long long*)&W.ps.stats[(blockIdx.x & (TPT_WF_SEGS - 1)) * TPT_STAT_STRIDE + 1],
+              (unsigned long long)tot);
+}
+
 // ---------------------------------------------------------------- host-side launchers
 // Grid of at most TPT_WF_MAX_BLOCKS blocks: the kernels loop over their queues, so the grid only
 // needs to fill the chip (256 CUs x 8 blocks); larger grids cost dispatch time per bounce,
@@ -518,8 +654,23 @@ static void launch_extend(const WfParams& W, int cur, const PassArgs& A, uint32_
   hipLaunchKernelGGL((k_extend<OVF, GEN>), grid_of(max_n, (const void*)k_extend<OVF, GEN>), dim3(TPT_WF_BLOCK), 0, s, W,
                      cur, A);
 }
+template <bool OVF, bool GEN>
+static void launch_extend_refill(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL((k_extend_refill<OVF, GEN>), grid_of(max_n, (const void*)k_extend_refill<OVF, GEN>),
+                     dim3(TPT_WF_BLOCK), 0, s, W, cur, A);
+}
 hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s) {
   const PassArgs A = gen ? *gen : PassArgs{};
+  if (W.refill > 0) {
+    if (W.gstack) {
+      if (gen) launch_extend_refill<true, true>(W, cur, A, max_n, s);
+      else launch_extend_refill<true, false>(W, cur, A, max_n, s);
+    } else {
+      if (gen) launch_extend_refill<false, true>(W, cur, A, max_n, s);
+      else launch_extend_refill<false, false>(W, cur, A, max_n, s);
+    }
+    return hipGetLastError();
+  }
   if (W.gstack) {
     if (gen) launch_extend<true, true>(W, cur, A, max_n, s);
     else launch_extend<true, false>(W, cur, A, max_n, s);
@@ -556,6 +707,15 @@ hipError_t wf_launch_light_selfocc(const WfParams& W, int cur, const PassArgs* g
   return hipGetLastError();
 }
 hipError_t wf_launch_shadow(const WfParams& W, int queue, uint32_t max_n, hipStream_t s) {
+  if (W.refill_sh > 0) {
+    if (W.gstack)
+      hipLaunchKernelGGL(k_shadow_refill<true>, grid_of(max_n, (const void*)k_shadow_refill<true>), dim3(TPT_WF_BLOCK),
+                         0, s, W, queue);
+    else
+      hipLaunchKernelGGL(k_shadow_refill<false>, grid_of(max_n, (const void*)k_shadow_refill<false>),
+                         dim3(TPT_WF_BLOCK), 0, s, W, queue);
+    return hipGetLastError();
+  }
   if (W.gstack)
     hipLaunchKernelGGL(k_shadow<true>, grid_of(max_n, (const void*)k_shadow<true>), dim3(TPT_WF_BLOCK), 0, s, W, queue);
   else

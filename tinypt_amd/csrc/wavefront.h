@@ -31,7 +31,9 @@ static_assert(TPT_WF_BLOCK % 64 == 0 && TPT_WF_BLOCK / 64 <= 7, "block_slot keep
 #define TPT_CNT_IDX(q, g) (((q) * TPT_WF_SEGS + (g)) * TPT_CNT_STRIDE)
 #define TPT_CNT_ACTIVE (5 * TPT_WF_SEGS * TPT_CNT_STRIDE)    // adaptive active-pixel list (one dense queue)
 #define TPT_CNT_OVERFLOW (TPT_CNT_ACTIVE + TPT_CNT_STRIDE)  // set if a segment would overflow (never by construction)
-#define TPT_NUM_CNT (TPT_CNT_OVERFLOW + TPT_CNT_STRIDE)
+#define TPT_CNT_FETCH (TPT_CNT_OVERFLOW + TPT_CNT_STRIDE)    // next ray of the refilling extension kernel
+#define TPT_CNT_FETCH_SH(q) (TPT_CNT_FETCH + (1 + (q)) * TPT_CNT_STRIDE)  // next ray of shadow queue q (refilling)
+#define TPT_NUM_CNT (TPT_CNT_FETCH + 3 * TPT_CNT_STRIDE)
 #define TPT_STAT_STRIDE 32  // uint64 words of statistics per segment (256 B)
 #define TPT_STAT_WORDS (TPT_WF_SEGS * TPT_STAT_STRIDE)
 static_assert(TPT_WF_MAX_BLOCKS % TPT_WF_SEGS == 0, "grids are multiples of TPT_WF_SEGS");
@@ -119,6 +121,8 @@ struct WfParams {
   int32_t max_bounces, rr_start;
   float inv_w, inv_h;
   uint64_t seed;
+  int32_t refill;     // extension waves take new rays once fewer than this many lanes trace (0: one ray per lane)
+  int32_t refill_sh;  // the same for the shadow kernels
 };
 
 // The paths of one pass: path i is pixel (i mod n_pixels) of the pass's pixel set, sample

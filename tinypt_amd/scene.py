@@ -366,6 +366,7 @@ class RenderConfig:
     seed: int = 0
     precision: int = _abi.FP32_WAVEFRONT
     samples_per_pass: int = 0
+    traversal_refill: int = 0     # 0: by scene shape, -1: off, 1..63: refill threshold (lanes)
     # host-side settings (config.rs:17-45, 79-86): checkpoints and image output
     checkpoint_enabled: bool = False
     checkpoint_every_tasks: int = 128
@@ -387,5 +388,5 @@ class RenderConfig:
         c.adaptive_threshold = float(self.adaptive_threshold)
         c.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
         c.samples_per_pass = int(self.samples_per_pass)
-        c.reserved = 0
+        c.traversal_refill = int(self.traversal_refill)
         return c
