@@ -42,6 +42,7 @@ extern "C" int tpt_test_host_trace(const tpt_scene_desc* d, int64_t n, const flo
   S.tlas_root = H.tlas_root;
   S.n_spheres = (int32_t)H.sph_mat.size();
   S.flat_inst = H.flat ? H.flat_inst : -1;
+  S.flat_root = H.flat_only_root();
   *flat = H.flat ? 1 : 0;
   // room far beyond the bound, so that a bound that is too small shows up in stack_max rather
   // than as a write past the buffer
@@ -108,6 +109,7 @@ extern "C" int tpt_test_selfocc(const tpt_scene_desc* d, int64_t n, const float*
   S.tlas_root = H.tlas_root;
   S.n_spheres = (int32_t)H.sph_mat.size();
   S.flat_inst = H.flat ? H.flat_inst : -1;
+  S.flat_root = H.flat_only_root();
   SceneF64 S64{};
   S64.nodes = H.ref_nodes.data();
   S64.indices = H.ref_indices.data();

@@ -270,6 +270,7 @@ int tpt_scene_upload(tpt_ctx* ctx, const tpt_scene_desc* desc, tpt_scene** out) 
   S.tri_orig = sc->tri_orig.as<int32_t>();
   S.tri_inst = H.flat ? sc->tri_inst.as<int32_t>() : nullptr;
   S.flat_inst = H.flat ? H.flat_inst : -1;
+  S.flat_root = H.flat_only_root();
   S.inst = sc->inst32.as<InstF32>();
   S.inst64 = sc->inst64.as<InstF64>();
   S.inst_c64 = sc->inst_c64.as<double>();

@@ -121,6 +121,12 @@ struct HostScene {
   bool build(const tpt_scene_desc* d);
   void rebuild_lights();
   size_t n_tris() const { return tri_mat.size(); }
+  // Root of the flattened world-space tree when the top level holds nothing else (no spheres):
+  // rays then start there, without the top-level node and the instance entry.  Else
+  // TPT_REF_EMPTY.
+  int32_t flat_only_root() const {
+    return flat && sph_mat.empty() ? inst32[(size_t)flat_inst].root : TPT_REF_EMPTY;
+  }
 };
 
 }  // namespace tpt

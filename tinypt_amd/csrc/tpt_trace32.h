@@ -46,7 +46,7 @@ struct SceneF32 {
   int32_t tlas_root;
   int32_t n_spheres;
   int32_t flat_inst;  // the pseudo-instance holding the flattened world-space tree, or -1
-  int32_t pad;
+  int32_t flat_root;  // that tree's root when the top level holds nothing else, or TPT_REF_EMPTY
 };
 
 struct HitF32 {
@@ -146,6 +146,11 @@ struct TraceF32 {
     found = false;
     ref = S.tlas_root;
     if (ref == TPT_REF_EMPTY) return false;
+    cur_inst = -1;
+    if (S.flat_root != TPT_REF_EMPTY) {  // start inside the flattened tree (world space)
+      ref = S.flat_root;
+      cur_inst = S.flat_inst;
+    }
     o = ro;
     d = rd;
     tmin = rtmin;
@@ -159,7 +164,6 @@ struct TraceF32 {
     det_eps = 1e-10f;
     best_t = rtmax;
     best_key = -1;
-    cur_inst = -1;
     cur_key = 0;
     sp = 0;
     return true;
