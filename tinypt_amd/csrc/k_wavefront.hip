@@ -409,6 +409,11 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W, int 
     if (j < n) {
       slot = qm.slot(j);
       const float4 l = W.ps.sh_l[slot];
+      // the light and its uniforms first: their loads overlap the hit point's chain below
+      RngF32 rng(1);
+      rng.p.state = W.ps.sh_lv[slot];
+      const int idx = light_pick32(W.lights, rng);
+      const double u = rng.uni53(), v = rng.uni53();
       const uint32_t pos = (uint32_t)__float_as_int(l.w);  // the path's dense position this bounce
       f3 o, d;
       if constexpr (GEN) {
@@ -421,10 +426,6 @@ __global__ void __launch_bounds__(TPT_WF_BLOCK) k_light_selfocc(WfParams W, int 
       }
       const HitRec hr = W.ps.hit[pos];
       const D3 p64 = hit_point64(W.S64, W.S.tri_orig, hr.inst, hr.prim, o, d, f3of(l));
-      RngF32 rng(1);
-      rng.p.state = W.ps.sh_lv[slot];
-      const int idx = light_pick32(W.lights, rng);
-      const double u = rng.uni53(), v = rng.uni53();
       keep = !light_self_occluded64(W.S64, W.lights, idx, p64, u, v);
     }
     const uint32_t k = seg_slot(keep, W.ps, TPT_CNT_SHK, s_slot);  // every thread of the block

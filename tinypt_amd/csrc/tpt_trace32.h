@@ -274,12 +274,14 @@ struct TraceF32 {
             const float4* w = reinterpret_cast<const float4*>(I.w2o);
             const float4 r0 = w[0], r1 = w[1], r2 = w[2];
             // re-base the origin in FP64 to o + s d, the ray's point nearest the instance's
-            // centre c (s rounded to FP32 first, so that the shift of t is exactly s)
+            // centre c (s is an FP32 number, so that the shift of t is exactly s; any s near
+            // the nearest point serves, so the division by |d|^2 is an FP32 reciprocal)
             {
               const double* c = S.inst_c64 + 3 * (size_t)item;
               const InstF64& J = S.inst64[item];
               const double dx = d.x, dy = d.y, dz = d.z, ox = o.x, oy = o.y, oz = o.z;
-              shift = (float)(((c[0] - ox) * dx + (c[1] - oy) * dy + (c[2] - oz) * dz) / (dx * dx + dy * dy + dz * dz));
+              const float inv_dd = 1.0f / (d.x * d.x + d.y * d.y + d.z * d.z);
+              shift = (float)((c[0] - ox) * dx + (c[1] - oy) * dy + (c[2] - oz) * dz) * inv_dd;
               const double s = shift;
               const double px = ox + s * dx - J.t[0], py = oy + s * dy - J.t[1], pz = oz + s * dz - J.t[2];
               const double* m = J.a_inv;
