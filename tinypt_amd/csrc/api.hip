@@ -444,18 +444,25 @@ int needed_gstack(const HostScene& H) {
 }
 
 // Lanes below which a wave of the persistent traversal kernels takes new rays (0: one ray per
-// lane, the plain kernels).  Refilling pays where the rays of a wave need very different
-// amounts of work and costs a global cursor atomic per refill.  Measured on MI355X (FP32 form,
-// images unchanged): a two-level scene of 4096 instances (top-level depth 16, 3.4M BLAS nodes)
-// at 48 -- extension -17%, shadow rays -9%, adaptive render 26.3 -> 20.1 s; a single tree of
-// 1.31M triangles at 16 -- -1% / -5%; the 36-triangle Cornell box at 16 -- extension 2.1x
-// slower (its rays finish within a few steps, so waves refill almost every step).
-int refill_threshold(const HostScene& H, int32_t requested) {
-  if (requested < 0) return 0;
-  if (requested > 0) return std::min(requested, 63);
-  if (!H.flat && H.inst32.size() >= 64) return 48;
-  if (H.stack4 > TPT_WF_STACK) return 16;
-  return 0;
+// lane, the plain kernels), for the first bounce's extension, the later extensions and the
+// shadow rays.  Refilling pays where the rays of a wave need very different amounts of work and
+// costs a global cursor atomic per refill.  Measured on MI355X (FP32 form, images unchanged): a
+// two-level scene of 4096 instances (top-level depth 16, 3.4M BLAS nodes) at 48 -- extension
+// -17%, shadow rays -9%, adaptive render 26.3 -> 19.0 s (its adaptive passes' camera rays, of
+// scattered active pixels, gain too); a single tree of 1.31M triangles at 16 -- later bounces
+// -5%, shadow rays -17%, but its first bounce, coherent camera rays, 75% slower, so that one
+// keeps one ray per lane; the 36-triangle Cornell box at 16 -- extension 2.1x slower (its rays
+// finish within a few steps, so waves refill almost every step).
+void refill_thresholds(const HostScene& H, int32_t requested, WfParams* W) {
+  int32_t first = 0, later = 0;
+  if (requested > 0) {
+    first = later = std::min(requested, 63);
+  } else if (requested == 0) {
+    if (!H.flat && H.inst32.size() >= 64) first = later = 48;
+    else if (H.stack4 > TPT_WF_STACK) later = 16;
+  }
+  W->refill_first = first;
+  W->refill = W->refill_sh = later;
 }
 
 // Queue arrays hold TPT_WF_SEGS segments of wf_seg_cap(P) entries; hit and L are dense.
@@ -516,7 +523,7 @@ WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, 
   ps.seg_cap = wf_seg_cap((uint32_t)c->ps_cap);
   W.gstack = gcap > 0 ? c->gstack.as<int32_t>() : nullptr;
   W.gstack_cap = gcap;
-  W.refill = W.refill_sh = refill_threshold(sc->host, cfg->traversal_refill);
+  refill_thresholds(sc->host, cfg->traversal_refill, &W);
   W.width = cfg->width;
   W.height = cfg->height;
   W.max_bounces = cfg->max_bounces;

@@ -130,21 +130,26 @@ k_extend(WfParams W, int cur, PassArgs A) {
   }
 }
 
-// Persistent form of k_extend for the bounces after the first on deep trees, where the rays of
-// one wave need very different numbers of traversal steps: a wave keeps its lanes' traversals
-// resumable (TraceF32 step by step) and, once fewer than W.refill lanes are still tracing, hands
-// its idle lanes the next rays of the queue (one atomic per refill on a global cursor; dynamic
-// ray fetch, Aila and Laine 2009).  The hits are those of k_extend: each ray's traversal is the same sequence of steps.
+// Persistent form of k_extend for deep trees, where the rays of one wave need very different
+// numbers of traversal steps: a wave keeps its lanes' traversals resumable (TraceF32 step by
+// step) and, once fewer than T lanes are still tracing (W.refill, or W.refill_first on the first
+// bounce), hands its idle lanes the next rays of the queue (one atomic per refill on a global
+// cursor; dynamic ray fetch, Aila and Laine 2009).  The hits are those of k_extend: each ray's traversal is the same sequence of steps.
 // The cursor (TPT_CNT_FETCH) is zero at launch: the render clears it, and every k_shade launch
 // clears it for the next extension launch.  Every wave exits once the cursor has passed the
 // queue's end and its lanes are idle.
-template <bool OVF>
-__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(6)))
-k_extend_refill(WfParams W, int cur) {
+template <bool OVF, bool GEN>
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(GEN ? 5 : 6)))
+k_extend_refill(WfParams W, int cur, PassArgs A) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
-  qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
-  const uint32_t n = qm.n;
+  uint32_t n;
+  if constexpr (GEN) {
+    n = A.n_paths;
+  } else {
+    qm.load(W.ps.counters, cur ? TPT_CNT_Q1 : TPT_CNT_Q0, W.ps.seg_cap);
+    n = qm.n;
+  }
   const float4* __restrict__ po = W.ps.o[cur];
   const float4* __restrict__ pd = W.ps.d[cur];
   LaneStackT<OVF> st{s_stack + threadIdx.x, OVF ? W.gstack + (size_t)gtid() * W.gstack_cap : nullptr, TPT_WF_BLOCK,
@@ -162,7 +167,8 @@ k_extend_refill(WfParams W, int cur) {
   TraceF32<false> tr;
   bool live = false, exhausted = false;
   uint32_t my = 0;
-  const int threshold = min(W.refill, 64);  // nl < threshold <= 64: every refill moves the cursor
+  // nl < threshold <= 64: every refill moves the cursor
+  const int threshold = min(GEN ? W.refill_first : W.refill, 64);
   for (;;) {
     const uint64_t mask = __ballot(live);
     const int nl = __popcll(mask);
@@ -174,8 +180,16 @@ k_extend_refill(WfParams W, int cur) {
       if (!live) {
         my = base + (uint32_t)__popcll(~mask & ((1ull << lane) - 1ull));
         if (my < n) {
-          const uint32_t slot = qm.slot(my);
-          live = tr.init(W.S, f3of(po[slot]), f3of(pd[slot]), (float)TPT_RAY_EPSILON, 1e30f);
+          f3 o, d;
+          if constexpr (GEN) {
+            RngF32 rng(1);
+            gen_path(W, A, my, &rng, &o, &d);
+          } else {
+            const uint32_t slot = qm.slot(my);
+            o = f3of(po[slot]);
+            d = f3of(pd[slot]);
+          }
+          live = tr.init(W.S, o, d, (float)TPT_RAY_EPSILON, 1e30f);
           if (!live) W.ps.hit[my] = HitRec{__float_as_int(tr.hit.t), tr.hit.inst, tr.hit.prim};
         }
       }
@@ -642,18 +656,21 @@ static void launch_extend(const WfParams& W, int cur, const PassArgs& A, uint32_
   hipLaunchKernelGGL((k_extend<OVF, GEN>), grid_of(max_n, (const void*)k_extend<OVF, GEN>), dim3(TPT_WF_BLOCK), 0, s, W,
                      cur, A);
 }
-template <bool OVF>
-static void launch_extend_refill(const WfParams& W, int cur, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL((k_extend_refill<OVF>), grid_of(max_n, (const void*)k_extend_refill<OVF>), dim3(TPT_WF_BLOCK), 0,
-                     s, W, cur);
+template <bool OVF, bool GEN>
+static void launch_extend_refill(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL((k_extend_refill<OVF, GEN>), grid_of(max_n, (const void*)k_extend_refill<OVF, GEN>),
+                     dim3(TPT_WF_BLOCK), 0, s, W, cur, A);
 }
 hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s) {
   const PassArgs A = gen ? *gen : PassArgs{};
-  // camera rays of neighbouring pixels need similar work, so the first bounce keeps one ray per
-  // lane (refilling it made the 1.31M-triangle mesh's first bounce 75% slower)
-  if (W.refill > 0 && !gen) {
-    if (W.gstack) launch_extend_refill<true>(W, cur, max_n, s);
-    else launch_extend_refill<false>(W, cur, max_n, s);
+  if (gen ? W.refill_first > 0 : W.refill > 0) {
+    if (W.gstack) {
+      if (gen) launch_extend_refill<true, true>(W, cur, A, max_n, s);
+      else launch_extend_refill<true, false>(W, cur, A, max_n, s);
+    } else {
+      if (gen) launch_extend_refill<false, true>(W, cur, A, max_n, s);
+      else launch_extend_refill<false, false>(W, cur, A, max_n, s);
+    }
     return hipGetLastError();
   }
   if (W.gstack) {

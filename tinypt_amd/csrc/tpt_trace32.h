@@ -76,6 +76,17 @@ using LaneStack = LaneStackT<true>;
 
 TPT_HD float safe_rcp(float d) { return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d)); }
 
+// 1 / x to about 2^-22 relative: the hardware reciprocal on the device (a correctly rounded FP64
+// division costs a dozen instructions and, in the traversal kernels, registers that spill);
+// the host build of the traversal divides.
+TPT_HD double approx_rcp64(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcp(x);
+#else
+  return 1.0 / x;
+#endif
+}
+
 #define TPT_KEY_SPHERE(s) (0x40000000 + (int32_t)(s))
 #define TPT_KEY_INST(i) (0x3fffffff - (int32_t)(i))
 #define TPT_STACK_SENTINEL TPT_REF_EMPTY
@@ -275,13 +286,13 @@ struct TraceF32 {
             const float4 r0 = w[0], r1 = w[1], r2 = w[2];
             // re-base the origin in FP64 to o + s d, the ray's point nearest the instance's
             // centre c (s is an FP32 number, so that the shift of t is exactly s; any s near
-            // the nearest point serves, so the division by |d|^2 is an FP32 reciprocal)
+            // the nearest point serves, so |d|^2 is divided by with an approximate reciprocal)
             {
               const double* c = S.inst_c64 + 3 * (size_t)item;
               const InstF64& J = S.inst64[item];
               const double dx = d.x, dy = d.y, dz = d.z, ox = o.x, oy = o.y, oz = o.z;
-              const float inv_dd = 1.0f / (d.x * d.x + d.y * d.y + d.z * d.z);
-              shift = (float)((c[0] - ox) * dx + (c[1] - oy) * dy + (c[2] - oz) * dz) * inv_dd;
+              shift = (float)(((c[0] - ox) * dx + (c[1] - oy) * dy + (c[2] - oz) * dz) *
+                              approx_rcp64(dx * dx + dy * dy + dz * dz));
               const double s = shift;
               const double px = ox + s * dx - J.t[0], py = oy + s * dy - J.t[1], pz = oz + s * dz - J.t[2];
               const double* m = J.a_inv;

@@ -121,8 +121,10 @@ struct WfParams {
   int32_t max_bounces, rr_start;
   float inv_w, inv_h;
   uint64_t seed;
-  int32_t refill;     // extension waves take new rays once fewer than this many lanes trace (0: one ray per lane)
-  int32_t refill_sh;  // the same for the shadow kernels
+  int32_t refill;        // extension waves take new rays once fewer than this many lanes trace (0: one ray per lane)
+  int32_t refill_first;  // the same for the first bounce's extension (its camera rays)
+  int32_t refill_sh;     // the same for the shadow kernels
+  int32_t pad;
 };
 
 // The paths of one pass: path i is pixel (i mod n_pixels) of the pass's pixel set, sample
