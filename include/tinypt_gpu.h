@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TPT_ABI_VERSION 2
+#define TPT_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------------------------ */
 typedef enum tpt_status {
@@ -228,16 +228,20 @@ int tpt_render_pixels(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_con
 /* Pass sizing of an FP32 wavefront render (host arithmetic; no device, no context).  The
  * wavefront form keeps pixels x samples paths in flight per pass.  For npix pixels and at most
  * path_budget paths in flight (the renders use 2^26, halved while device memory is short):
- * paths_alloc = path slots the render allocates; samples_per_pass / pixels_per_pass = the
- * fixed-spp passes, or, when cfg->adaptive_enabled, the adaptive pass that starts after `taken`
- * samples with n_active pixels still sampling (its active list is launched in chunks of
- * pixels_per_pass pixels).  No pass holds more than paths_alloc paths.  No reference
+ * paths_alloc = path slots of one path set; path_sets = the sets the render allocates (2 for
+ * a fixed-spp render of two or more passes, which alternate between the sets, each then at
+ * most half the budget; else 1); samples_per_pass / pixels_per_pass = the fixed-spp passes,
+ * or, when cfg->adaptive_enabled, the adaptive pass that starts after `taken` samples with
+ * n_active pixels still sampling (its active list is launched in chunks of pixels_per_pass
+ * pixels).  No pass holds more than paths_alloc paths.  No reference
  * counterpart: the reference's worker threads take one sample of one pixel at a time
  * (render.rs:131-191). */
 typedef struct tpt_pass_plan {
   uint64_t paths_alloc;
   uint32_t samples_per_pass;
   uint32_t pixels_per_pass;
+  uint32_t path_sets;
+  uint32_t reserved;
 } tpt_pass_plan;
 int tpt_render_plan(const tpt_render_config* cfg, int64_t npix, uint32_t path_budget, uint32_t n_active,
                     uint32_t taken, tpt_pass_plan* out);

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), f"{n} declared in the header but not exported"
         assert n in _abi.ENTRY_POINTS, f"{n} has no ctypes signature"
     assert set(_abi.ENTRY_POINTS) == set(names)
-    assert lib.tpt_abi_version() == 2
+    assert lib.tpt_abi_version() == _abi.ABI_VERSION == 3
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -38,9 +38,11 @@ def test_struct_layouts_match_header(tmp_path):
         "tpt_camera_desc": (_abi.CameraDesc, ["eye", "vfov_deg", "aperture"]),
         "tpt_scene_desc": (_abi.SceneDesc, ["camera", "n_materials", "mat_param", "n_spheres", "mesh_tri_begin",
                                             "tri_mat", "inst_mat_override", "env_kind", "env_rgb"]),
-        "tpt_render_config": (_abi.RenderConfigC, ["spp", "precision", "adaptive_threshold", "seed", "samples_per_pass"]),
+        "tpt_render_config": (_abi.RenderConfigC, ["spp", "precision", "adaptive_threshold", "seed", "samples_per_pass",
+                                                   "traversal_refill"]),
         "tpt_stats": (_abi.Stats, ["paths", "elapsed_ms", "trace_rays", "light_self_occluded"]),
         "tpt_scene_info": (_abi.SceneInfo, ["n_tris", "max_tlas_depth", "device_bytes"]),
+        "tpt_pass_plan": (_abi.PassPlan, ["paths_alloc", "samples_per_pass", "pixels_per_pass", "path_sets"]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
     for s, (_, fields) in structs.items():

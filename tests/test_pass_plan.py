@@ -11,12 +11,17 @@ import tinypt_amd as T
 from tinypt_amd import _abi
 
 
-def plan(cfg, npix, budget, n_active=0, taken=0):
+def plan_full(cfg, npix, budget, n_active=0, taken=0):
     out = _abi.PassPlan()
     c = cfg.to_c()
     rc = T.load_library().tpt_render_plan(C.byref(c), npix, budget, n_active, taken, C.byref(out))
     if rc:
         raise T.TptError(rc, T.load_library().tpt_last_error().decode())
+    return out
+
+
+def plan(cfg, npix, budget, n_active=0, taken=0):
+    out = plan_full(cfg, npix, budget, n_active, taken)
     return out.paths_alloc, out.samples_per_pass, out.pixels_per_pass
 
 
@@ -31,10 +36,13 @@ def cfg_for(w, h, spp, adaptive=False, min_spp=64, spp_pass=0):
     (16384, 16384, 2, 64, 1 << 26), (1, 1, 0, 0, 1 << 26)])
 def test_fixed_spp_passes_fit_and_cover(w, h, spp, spp_pass, budget):
     npix = w * h
-    alloc, k, chunk = plan(cfg_for(w, h, spp, spp_pass=spp_pass), npix, budget)
+    full = plan_full(cfg_for(w, h, spp, spp_pass=spp_pass), npix, budget)
+    alloc, k, chunk, sets = full.paths_alloc, full.samples_per_pass, full.pixels_per_pass, full.path_sets
     assert k >= 1 and chunk >= 1
     assert chunk * k <= alloc
-    assert alloc <= 2 * budget
+    assert sets in (1, 2) and sets * alloc <= 2 * budget
+    n_passes = -(-npix // chunk) * -(-spp // k)
+    assert (sets == 2) == (n_passes >= 2), (sets, n_passes)
     # the loops of the render: pixel chunks x sample groups cover every (pixel, sample) once
     covered = 0
     for p0 in range(0, npix, chunk):
@@ -90,3 +98,16 @@ def test_plan_rejects_bad_arguments():
         plan(cfg_for(4, 4, 8, adaptive=True), 16, 1 << 20, n_active=17)
     with pytest.raises(T.TptError):
         plan(cfg_for(4, 4, 8), 0, 1 << 20)
+
+
+def test_two_path_sets_for_renders_of_several_passes():
+    # the Cornell benchmark (512^2 x 256 spp, 2^26 paths in flight): two passes of 128 samples,
+    # one per path set of 2^25 paths
+    full = plan_full(cfg_for(512, 512, 256), 512 * 512, 1 << 26)
+    assert (full.path_sets, full.samples_per_pass, full.pixels_per_pass, full.paths_alloc) == (2, 128, 512 * 512, 1 << 25)
+    # a render that fits one pass keeps one set of the whole budget
+    full = plan_full(cfg_for(256, 256, 64), 256 * 256, 1 << 26)
+    assert (full.path_sets, full.samples_per_pass, full.paths_alloc) == (1, 64, 256 * 256 * 64)
+    # adaptive renders: one set (their passes depend on each other)
+    full = plan_full(cfg_for(512, 512, 256, adaptive=True), 512 * 512, 1 << 26, n_active=512 * 512)
+    assert full.path_sets == 1

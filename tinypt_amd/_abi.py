@@ -109,6 +109,8 @@ class PassPlan(C.Structure):
         ("paths_alloc", C.c_uint64),
         ("samples_per_pass", C.c_uint32),
         ("pixels_per_pass", C.c_uint32),
+        ("path_sets", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -165,6 +167,6 @@ ENTRY_POINTS = {
     "tpt_checkpoint_load": (C.c_int, [C.c_char_p, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_uint64), _dp, _dp,
                                       _ip]),
 }
-ABI_VERSION = 2  # TPT_ABI_VERSION of include/tinypt_gpu.h
+ABI_VERSION = 3  # TPT_ABI_VERSION of include/tinypt_gpu.h
 TONEMAP_NONE = 0
 TONEMAP_ACES = 1

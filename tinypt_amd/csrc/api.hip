@@ -84,20 +84,30 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
 
 }  // namespace
 
+// One set of wavefront path state (grow-only) with the stream its passes run on.  A context has
+// two: passes of a non-adaptive render alternate between them, so one set's kernels fill the
+// ramp-down of the other's.
+struct PathSet {
+  hipStream_t stream = nullptr;
+  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o[2], sh_d[2], sh_c[2], sh_l, sh_lv, shl_keep, counters,
+      stats, gstack;
+  size_t ps_cap = 0;
+  EventPool ev_trace;  // timing events of the extension launches (one call at a time)
+  hipEvent_t done = nullptr;  // recorded after the set's last accumulation of a render
+};
+
 struct tpt_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
-  // wavefront scratch (grow-only)
-  DevBuf ps_o[2], ps_d[2], ps_beta[2], ps_rng[2], ps_hit, ps_L, sh_o[2], sh_d[2], sh_c[2], sh_l, sh_lv, shl_keep, counters, stats,
-      gstack;
+  hipStream_t stream = nullptr;  // the context's stream (set[0]'s)
+  PathSet set[2];
+  DevBuf stats;  // plain FP64 form counters
   // adaptive / plain per-pixel state
   DevBuf act0, act1, nsamp, welford, rngpix, done;
   // adjoint (grow-only, kept between calls)
   DevBuf adj[ADJ_SCRATCH_N];
   // device sums of renders into host buffers (grow-only)
   DevBuf out_acc, out_w, out_sq;
-  size_t ps_cap = 0;
-  EventPool ev_total, ev_trace;  // timing events of the render calls (one call at a time)
+  EventPool ev_total;  // timing events of the render calls (one call at a time)
 };
 
 struct tpt_scene {
@@ -189,6 +199,9 @@ int tpt_ctx_create(int device, tpt_ctx** out) {
   std::unique_ptr<tpt_ctx> c(new tpt_ctx());
   c->device = device;
   HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->set[0].stream = c->stream;
+  HIPC(hipStreamCreateWithFlags(&c->set[1].stream, hipStreamNonBlocking));
+  for (PathSet& ps : c->set) HIPC(hipEventCreateWithFlags(&ps.done, hipEventDisableTiming));
   *out = c.release();
   return TPT_OK;
 }
@@ -196,7 +209,11 @@ int tpt_ctx_create(int device, tpt_ctx** out) {
 int tpt_ctx_destroy(tpt_ctx* ctx) {
   if (!ctx) return TPT_OK;
   (void)hipSetDevice(ctx->device);  // teardown: errors cannot be acted on
-  (void)hipStreamSynchronize(ctx->stream);
+  for (PathSet& ps : ctx->set) {
+    if (ps.stream) (void)hipStreamSynchronize(ps.stream);
+    if (ps.done) (void)hipEventDestroy(ps.done);
+  }
+  if (ctx->set[1].stream) (void)hipStreamDestroy(ctx->set[1].stream);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return TPT_OK;
@@ -466,7 +483,7 @@ void refill_thresholds(const HostScene& H, int32_t requested, WfParams* W) {
 }
 
 // Queue arrays hold TPT_WF_SEGS segments of wf_seg_cap(P) entries; hit and L are dense.
-int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
+int ensure_paths(PathSet* c, size_t P, int gcap) {
   if (P > c->ps_cap) {
     const size_t Q = (size_t)TPT_WF_SEGS * wf_seg_cap((uint32_t)P);
     for (int b = 0; b < 2; b++) {
@@ -493,7 +510,7 @@ int ensure_paths(tpt_ctx* c, size_t P, int gcap) {
   return TPT_OK;
 }
 
-WfParams make_wf(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* cfg, int gcap) {
+WfParams make_wf(PathSet* c, const tpt_scene* sc, const tpt_render_config* cfg, int gcap) {
   WfParams W{};
   W.S = sc->s32;
   W.S64 = sc->s64;
@@ -564,7 +581,7 @@ struct EventTimer {
 
 // Bounce loop of one pass over the A.n_paths paths that A describes; the first bounce's kernels
 // generate the camera rays themselves.
-int wf_bounces(tpt_ctx* c, const WfParams& W, const PassArgs& A, EventTimer& tt, uint64_t* launches) {
+int wf_bounces(PathSet* c, const WfParams& W, const PassArgs& A, EventTimer& tt, uint64_t* launches) {
   hipStream_t s = c->stream;
   const uint32_t P = A.n_paths;
   if ((size_t)P > c->ps_cap) return fail(TPT_ERR_INTERNAL, "a pass holds more paths than were allocated");
@@ -593,43 +610,71 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
                      const int32_t* pix_map, double* acc, double* acc_w, double* acc_sq, tpt_stats* st) {
   hipStream_t s = c->stream;
   const uint32_t spp = (uint32_t)cfg->spp;
-  // Paths in flight: up to 2^26 (about 18 GB of path state and queues) so that a 512^2 x 256 spp
-  // render is one pass with one drain of the bounce loop; less when the device has less free.
+  const bool adaptive = cfg->adaptive_enabled != 0;
+  // Paths in flight: up to 2^26 (about 18 GB of path state and queues), so that a 512^2 x 256
+  // spp render is one drain of the bounce loop per path set; less when the device has less free.
   uint32_t target = 1u << 26;
   {
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && c->ps_cap < target) {
+    const size_t have = c->set[0].ps_cap + c->set[1].ps_cap;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && have < target) {
       const size_t per_path = 280;  // both path sets, hit, L, both shadow queues, segment slack
       while (target > (1u << 20) && (size_t)target * per_path > free_b / 2) target >>= 1;
     }
   }
-  const PassPlan plan = plan_render(npix, spp, cfg->samples_per_pass, cfg->adaptive_enabled != 0, target);
+  // A non-adaptive render of two or more passes alternates them between the two path sets,
+  // each holding half the paths: a set's kernels then fill the other's ramp-downs (measured on
+  // the Cornell box: two 128-spp renders on two streams 30.4 ms, one after the other 32.0 ms).
+  // Accumulation stays in pass order, so the sums do not depend on the overlap.
+  const RenderPlan rp = plan_render_sets(npix, spp, cfg->samples_per_pass, adaptive, target);
+  const PassPlan plan = rp.pass;
+  const int nsets = rp.sets;
   const uint32_t k = plan.k, pix_chunk = plan.pix_chunk;
   const int gcap = needed_gstack(sc->host);
   const size_t P = (size_t)plan.paths_alloc;
-  int rc = ensure_paths(c, P, gcap);
-  if (rc) return rc;
-  WfParams W = make_wf(c, sc, cfg, gcap);
-  HIPC(hipMemsetAsync(W.ps.stats, 0, TPT_STAT_WORDS * 8, s));
-  HIPC(hipMemsetAsync(W.ps.counters, 0, TPT_NUM_CNT * 4, s));
-  EventTimer total(s, st != nullptr, &c->ev_total), tt(s, st != nullptr, &c->ev_trace);
-  uint64_t launches = 0, traced = 0;
+  WfParams W[2];
+  for (int j = 0; j < nsets; j++) {
+    PathSet* S = &c->set[j];
+    int rc = ensure_paths(S, P, gcap);
+    if (rc) return rc;
+    W[j] = make_wf(S, sc, cfg, gcap);
+    HIPC(hipMemsetAsync(W[j].ps.stats, 0, TPT_STAT_WORDS * 8, S->stream));
+    HIPC(hipMemsetAsync(W[j].ps.counters, 0, TPT_NUM_CNT * 4, S->stream));
+  }
+  EventTimer total(s, st != nullptr, &c->ev_total);
+  EventTimer tt0(c->set[0].stream, st != nullptr, &c->set[0].ev_trace);
+  EventTimer tt1(c->set[1].stream, st != nullptr, &c->set[1].ev_trace);
+  EventTimer* tt[2] = {&tt0, &tt1};
+  uint64_t launches = 0;
+  int rc = TPT_OK;
   HIPC(total.mark());
-  if (!cfg->adaptive_enabled) {
+  if (nsets == 2) {  // set 1 starts after the work already queued on the context stream
+    HIPC(hipEventRecord(c->set[0].done, s));
+    HIPC(hipStreamWaitEvent(c->set[1].stream, c->set[0].done, 0));
+  }
+  if (!adaptive) {
+    uint64_t pass = 0;
     for (uint32_t p0 = 0; p0 < npix; p0 += pix_chunk) {
       uint32_t np = std::min(pix_chunk, npix - p0);
       for (uint32_t s0 = 0; s0 < spp; s0 += k) {
+        const int j = nsets == 2 ? (int)(pass & 1) : 0;
+        PathSet* S = &c->set[j];
         uint32_t kk = std::min(k, spp - s0);
         PassArgs A{np * kk, np, p0, s0, nullptr, nullptr, pix_map};
-        rc = wf_bounces(c, W, A, tt, &launches);
+        rc = wf_bounces(S, W[j], A, *tt[j], &launches);
         if (rc) return rc;
-        traced += (uint64_t)np * kk;
         AccumArgs B{np, p0, kk, acc, acc_w, acc_sq};
-        HIPC(wf_launch_accumulate(W, B, s));
+        // after the previous pass's accumulation (queued on the other set's stream)
+        if (nsets == 2 && pass > 0) HIPC(hipStreamWaitEvent(S->stream, c->set[j ^ 1].done, 0));
+        HIPC(wf_launch_accumulate(W[j], B, S->stream));
+        if (nsets == 2) HIPC(hipEventRecord(S->done, S->stream));
+        pass++;
       }
     }
+    if (nsets == 2) HIPC(hipStreamWaitEvent(s, c->set[1].done, 0));
   } else {
     // render.rs:143-179 per pixel; each pass gives every active pixel k more samples
+    PathSet* S = &c->set[0];
     HIPC(c->act0.ensure((size_t)npix * 4));
     HIPC(c->act1.ensure((size_t)npix * 4));
     HIPC(c->nsamp.ensure((size_t)npix * 4));
@@ -648,19 +693,18 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
     while (n_active > 0 && taken < spp) {
       uint32_t kk = 1, chunk = 1;
       plan_adaptive_pass(n_active, taken, min_spp, spp, P, &kk, &chunk);
-      uint32_t* n_out = W.ps.counters + TPT_CNT_ACTIVE;
+      uint32_t* n_out = W[0].ps.counters + TPT_CNT_ACTIVE;
       HIPC(hipMemsetAsync(n_out, 0, 4, s));
       // the active list in chunks of at most P / kk pixels; survivors of every chunk are
       // appended to act_out (a pixel's samples do not depend on its position in the list)
       for (uint32_t c0 = 0; c0 < n_active; c0 += chunk) {
         const uint32_t nc = std::min(chunk, n_active - c0);
         PassArgs A{nc * kk, nc, 0, 0, act_in + c0, c->nsamp.as<uint32_t>(), pix_map};
-        rc = wf_bounces(c, W, A, tt, &launches);
+        rc = wf_bounces(S, W[0], A, tt0, &launches);
         if (rc) return rc;
-        traced += (uint64_t)nc * kk;
         AdaptiveArgs B{nc, kk, min_spp, spp, cfg->adaptive_threshold, act_in + c0, act_out, n_out,
                        c->nsamp.as<uint32_t>(), c->welford.as<double>(), acc, acc_w, acc_sq};
-        HIPC(wf_launch_accumulate_adaptive(W, B, s));
+        HIPC(wf_launch_accumulate_adaptive(W[0], B, s));
       }
       HIPC(hipMemcpyAsync(&n_active, n_out, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipStreamSynchronize(s));
@@ -670,26 +714,29 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   }
   HIPC(total.mark());
   HIPC(hipStreamSynchronize(s));
-  uint32_t overflow = 0;
-  HIPC(hipMemcpy(&overflow, W.ps.counters + TPT_CNT_OVERFLOW, 4, hipMemcpyDeviceToHost));
-  if (overflow) return fail(TPT_ERR_INTERNAL, "a path queue segment overflowed");
+  uint64_t h[5] = {0, 0, 0, 0, 0};
+  for (int j = 0; j < nsets; j++) {
+    uint32_t overflow = 0;
+    HIPC(hipMemcpy(&overflow, W[j].ps.counters + TPT_CNT_OVERFLOW, 4, hipMemcpyDeviceToHost));
+    if (overflow) return fail(TPT_ERR_INTERNAL, "a path queue segment overflowed");
+    if (st) {
+      std::vector<uint64_t> seg(TPT_STAT_WORDS);
+      HIPC(hipMemcpy(seg.data(), W[j].ps.stats, TPT_STAT_WORDS * 8, hipMemcpyDeviceToHost));
+      for (int g = 0; g < TPT_WF_SEGS; g++)
+        for (int q = 0; q < 5; q++) h[q] += seg[(size_t)g * TPT_STAT_STRIDE + q];
+    }
+  }
   if (st) {
-    std::vector<uint64_t> seg(TPT_STAT_WORDS);
-    HIPC(hipMemcpy(seg.data(), W.ps.stats, TPT_STAT_WORDS * 8, hipMemcpyDeviceToHost));
-    uint64_t h[5] = {0, 0, 0, 0, 0};
-    for (int g = 0; g < TPT_WF_SEGS; g++)
-      for (int k = 0; k < 5; k++) h[k] += seg[(size_t)g * TPT_STAT_STRIDE + k];
     st->paths = h[2];
     st->rays_extension = h[0];
     st->rays_shadow = h[1];
     st->elapsed_ms = total.pairs_ms();
-    st->trace_kernel_ms = tt.pairs_ms();
+    st->trace_kernel_ms = tt0.pairs_ms() + tt1.pairs_ms();
     st->trace_launches = launches;
     st->trace_rays = h[0];
     st->light_samples = h[3];
     st->light_self_occluded = h[4];
   }
-  (void)traced;
   return TPT_OK;
 }
 
@@ -843,9 +890,11 @@ int tpt_render_plan(const tpt_render_config* cfg, int64_t npix, uint32_t path_bu
   if (rc) return rc;
   if (npix <= 0 || npix > (int64_t)1 << 28) return fail(TPT_ERR_INVALID_ARGUMENT, "pixel count out of range");
   const uint32_t spp = (uint32_t)cfg->spp;
-  const PassPlan plan = plan_render((uint32_t)npix, spp, cfg->samples_per_pass, cfg->adaptive_enabled != 0,
-                                    path_budget);
+  const RenderPlan rp = plan_render_sets((uint32_t)npix, spp, cfg->samples_per_pass, cfg->adaptive_enabled != 0,
+                                         path_budget);
+  const PassPlan plan = rp.pass;
   out->paths_alloc = plan.paths_alloc;
+  out->path_sets = (uint32_t)rp.sets;
   out->samples_per_pass = plan.k;
   out->pixels_per_pass = plan.pix_chunk;
   if (cfg->adaptive_enabled) {

@@ -35,6 +35,23 @@ inline PassPlan plan_render(uint32_t npix, uint32_t spp, int32_t samples_per_pas
   return PassPlan{std::max<uint64_t>(P, 1), k, std::max(pix_chunk, 1u)};
 }
 
+// The render's plan: a fixed-spp render of two or more passes alternates them between two path
+// sets, each of at most half the budget (one set's kernels then fill the other's ramp-downs);
+// an adaptive render, whose passes depend on each other, and a one-pass render use one set of
+// the whole budget.
+struct RenderPlan {
+  PassPlan pass;
+  int sets;  // path sets of pass.paths_alloc slots each
+};
+inline RenderPlan plan_render_sets(uint32_t npix, uint32_t spp, int32_t samples_per_pass, bool adaptive,
+                                   uint32_t target) {
+  if (adaptive) return RenderPlan{plan_render(npix, spp, samples_per_pass, true, target), 1};
+  const PassPlan half = plan_render(npix, spp, samples_per_pass, false, std::max(target / 2, 1u));
+  const uint64_t passes = (uint64_t)((npix + half.pix_chunk - 1) / half.pix_chunk) * ((spp + half.k - 1) / half.k);
+  if (passes >= 2) return RenderPlan{half, 2};
+  return RenderPlan{plan_render(npix, spp, samples_per_pass, false, target), 1};
+}
+
 // One adaptive pass: n_active pixels are still sampling after `taken` samples each.  Every
 // active pixel gets *kk more samples (at least the rest of the min_spp phase, else up to 64 as
 // the slots allow); the active list is launched in chunks of *chunk pixels, with
