@@ -346,7 +346,8 @@ def main():
     workloads = None if args.no_workloads else other_workloads(ds.ctx, heavy=not args.no_heavy)
 
     # roofline of the extension kernel: its compulsory bytes per launch over its measured launch
-    # time (HIP events around every launch, on the render's stream)
+    # time (HIP events around every launch, on the stream of the path set that launches it; the
+    # render's two path sets run side by side, so a launch shares the GPU with the other set's)
     avg_launch_ms = trace_ms / max(trace_launches, 1)
     rays_per_launch = trace_rays / max(trace_launches, 1)
     first_rays = float(paths)  # one camera ray per path on the first bounce
@@ -362,6 +363,8 @@ def main():
                 "traffic_source": TRAFFIC_PROFILE if traffic else None,
                 "hbm_gbs_from_traffic": traffic / (avg_launch_ms * 1e-3) / 1e9 if traffic else None,
                 "limiter": {"kind": "VALU issue (divergent per-ray traversal work)", **(measured_valu() or {})},
+                "concurrency": "two path sets on two streams: each launch's duration includes the other set's "
+                               "kernels running beside it",
                 "reference_traversal_bytes_per_ray": bytes_per_ray}
 
     line = {"metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

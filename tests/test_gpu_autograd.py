@@ -68,7 +68,7 @@ def test_inverse_problem_reduces_image_error(gpu_ctx):
     determined (F0 and roughness trade off; the light has little blue), but the fit in image
     space must improve -- the error of a clean render above the noise floor shrinks."""
     prob = InverseProblem(width=128, height=128, target_spp=512, spp=16, ctx=gpu_ctx)
-    hist = optimize(prob, steps=100, lr=0.02, eval_spp=256)
+    hist = optimize(prob, steps=200, lr=0.02, eval_spp=256)
     floor = hist["image_error_truth"]
     before, after = hist["image_error_init"] - floor, hist["image_error_final"] - floor
     print(f"inverse 128x128 (16 spheres): image error above the floor {before:.3g} -> {after:.3g}", flush=True)
