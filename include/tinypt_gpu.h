@@ -230,10 +230,11 @@ int tpt_render_pixels(tpt_ctx* ctx, const tpt_scene* scene, const tpt_render_con
  * path_budget paths in flight (the renders use 2^26, halved while device memory is short):
  * paths_alloc = path slots of one path set; path_sets = the sets the render allocates (2 for
  * a fixed-spp render of two or more passes, which alternate between the sets, each then at
- * most half the budget; else 1); samples_per_pass / pixels_per_pass = the fixed-spp passes,
- * or, when cfg->adaptive_enabled, the adaptive pass that starts after `taken` samples with
- * n_active pixels still sampling (its active list is launched in chunks of pixels_per_pass
- * pixels).  No pass holds more than paths_alloc paths.  No reference
+ * most half the budget, and for an adaptive render, whose chunks alternate between them; else
+ * 1); samples_per_pass / pixels_per_pass = the fixed-spp passes, or, when
+ * cfg->adaptive_enabled, the adaptive pass that starts after `taken` samples with n_active
+ * pixels still sampling (its active list is launched in chunks of pixels_per_pass pixels).  No
+ * pass or chunk holds more than paths_alloc paths.  No reference
  * counterpart: the reference's worker threads take one sample of one pixel at a time
  * (render.rs:131-191). */
 typedef struct tpt_pass_plan {

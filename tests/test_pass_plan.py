@@ -78,10 +78,11 @@ def simulate_adaptive(w, h, spp, min_spp, budget, stop_rule):
 def test_adaptive_passes_fit_when_pixels_exceed_the_budget():
     # 16384 x 8193 pixels (accepted: below 2^28) with 2^26 paths in flight: before the chunked
     # active list, a pass launched one path per pixel into 2^26 slots
+    # (two path sets of half the slots each; a pass's chunks alternate between them)
     passes, alloc = simulate_adaptive(16384, 8193, 8, 2, 1 << 26, lambda t, n: n - n // 3)
-    assert alloc == 1 << 26 and passes >= 2
+    assert alloc == 1 << 25 and passes >= 2
     passes, alloc = simulate_adaptive(4096, 4096, 16, 4, 1 << 20, lambda t, n: n // 2)
-    assert alloc == 1 << 20
+    assert alloc == 1 << 19
 
 
 @pytest.mark.parametrize("w,h,spp,min_spp", [(1920, 1080, 4096, 64), (512, 512, 256, 64), (32, 32, 256, 64),
@@ -108,6 +109,10 @@ def test_two_path_sets_for_renders_of_several_passes():
     # a render that fits one pass keeps one set of the whole budget
     full = plan_full(cfg_for(256, 256, 64), 256 * 256, 1 << 26)
     assert (full.path_sets, full.samples_per_pass, full.paths_alloc) == (1, 64, 256 * 256 * 64)
-    # adaptive renders: one set (their passes depend on each other)
+    # adaptive renders: two sets of half the slots; a pass of 2^21 paths or more has at least two
+    # chunks, so both sets work (its samples per pixel are sized by the whole budget)
     full = plan_full(cfg_for(512, 512, 256, adaptive=True), 512 * 512, 1 << 26, n_active=512 * 512)
-    assert full.path_sets == 1
+    assert full.path_sets == 2 and full.paths_alloc == 1 << 25
+    assert full.samples_per_pass == 64 and full.pixels_per_pass == 512 * 512 // 2
+    full = plan_full(cfg_for(512, 512, 256, adaptive=True), 512 * 512, 1 << 26, n_active=1000, taken=64)
+    assert full.samples_per_pass == 64 and full.pixels_per_pass == 1000

@@ -625,7 +625,8 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   // A non-adaptive render of two or more passes alternates them between the two path sets,
   // each holding half the paths: a set's kernels then fill the other's ramp-downs (measured on
   // the Cornell box: two 128-spp renders on two streams 30.4 ms, one after the other 32.0 ms).
-  // Accumulation stays in pass order, so the sums do not depend on the overlap.
+  // Accumulation stays in pass order, so the sums do not depend on the overlap.  An adaptive
+  // pass's chunks alternate between the sets too; they hold disjoint pixels.
   const RenderPlan rp = plan_render_sets(npix, spp, cfg->samples_per_pass, adaptive, target);
   const PassPlan plan = rp.pass;
   const int nsets = rp.sets;
@@ -648,7 +649,7 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
   uint64_t launches = 0;
   int rc = TPT_OK;
   HIPC(total.mark());
-  if (nsets == 2) {  // set 1 starts after the work already queued on the context stream
+  if (nsets == 2 && !adaptive) {  // set 1 starts after the work already queued on the context stream
     HIPC(hipEventRecord(c->set[0].done, s));
     HIPC(hipStreamWaitEvent(c->set[1].stream, c->set[0].done, 0));
   }
@@ -674,7 +675,6 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
     if (nsets == 2) HIPC(hipStreamWaitEvent(s, c->set[1].done, 0));
   } else {
     // render.rs:143-179 per pixel; each pass gives every active pixel k more samples
-    PathSet* S = &c->set[0];
     HIPC(c->act0.ensure((size_t)npix * 4));
     HIPC(c->act1.ensure((size_t)npix * 4));
     HIPC(c->nsamp.ensure((size_t)npix * 4));
@@ -690,22 +690,30 @@ int render_wavefront(tpt_ctx* c, const tpt_scene* sc, const tpt_render_config* c
     const uint32_t min_spp = std::min((uint32_t)std::max(cfg->adaptive_min_spp, 1), max_spp);
     int32_t* act_in = c->act0.as<int32_t>();
     int32_t* act_out = c->act1.as<int32_t>();
+    // the survivors of every chunk, on either set, are appended to act_out through one counter
+    uint32_t* n_out = W[0].ps.counters + TPT_CNT_ACTIVE;
     while (n_active > 0 && taken < spp) {
       uint32_t kk = 1, chunk = 1;
-      plan_adaptive_pass(n_active, taken, min_spp, spp, P, &kk, &chunk);
-      uint32_t* n_out = W[0].ps.counters + TPT_CNT_ACTIVE;
+      plan_adaptive_pass(n_active, taken, min_spp, spp, rp.pass_cap, P, &kk, &chunk);
       HIPC(hipMemsetAsync(n_out, 0, 4, s));
-      // the active list in chunks of at most P / kk pixels; survivors of every chunk are
-      // appended to act_out (a pixel's samples do not depend on its position in the list)
-      for (uint32_t c0 = 0; c0 < n_active; c0 += chunk) {
+      // set 1 starts after this pass's set-up on the context stream
+      HIPC(hipEventRecord(c->set[0].done, s));
+      HIPC(hipStreamWaitEvent(c->set[1].stream, c->set[0].done, 0));
+      // the active list in chunks of at most P / kk pixels, alternating between the sets (a
+      // pixel's samples do not depend on its chunk or its position in the list)
+      int j = 0;
+      for (uint32_t c0 = 0; c0 < n_active; c0 += chunk, j ^= 1) {
+        PathSet* S = &c->set[j];
         const uint32_t nc = std::min(chunk, n_active - c0);
         PassArgs A{nc * kk, nc, 0, 0, act_in + c0, c->nsamp.as<uint32_t>(), pix_map};
-        rc = wf_bounces(S, W[0], A, tt0, &launches);
+        rc = wf_bounces(S, W[j], A, *tt[j], &launches);
         if (rc) return rc;
         AdaptiveArgs B{nc, kk, min_spp, spp, cfg->adaptive_threshold, act_in + c0, act_out, n_out,
                        c->nsamp.as<uint32_t>(), c->welford.as<double>(), acc, acc_w, acc_sq};
-        HIPC(wf_launch_accumulate_adaptive(W[0], B, s));
+        HIPC(wf_launch_accumulate_adaptive(W[j], B, S->stream));
       }
+      HIPC(hipEventRecord(c->set[1].done, c->set[1].stream));
+      HIPC(hipStreamWaitEvent(s, c->set[1].done, 0));
       HIPC(hipMemcpyAsync(&n_active, n_out, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipStreamSynchronize(s));
       taken += kk;
@@ -901,7 +909,7 @@ int tpt_render_plan(const tpt_render_config* cfg, int64_t npix, uint32_t path_bu
     if (n_active == 0 || n_active > npix) return fail(TPT_ERR_INVALID_ARGUMENT, "n_active must be in 1..npix");
     const uint32_t max_spp = std::max(spp, 1u);
     const uint32_t min_spp = std::min((uint32_t)std::max(cfg->adaptive_min_spp, 1), max_spp);
-    plan_adaptive_pass(n_active, taken, min_spp, spp, plan.paths_alloc, &out->samples_per_pass,
+    plan_adaptive_pass(n_active, taken, min_spp, spp, rp.pass_cap, plan.paths_alloc, &out->samples_per_pass,
                        &out->pixels_per_pass);
   }
   return TPT_OK;

@@ -35,38 +35,50 @@ inline PassPlan plan_render(uint32_t npix, uint32_t spp, int32_t samples_per_pas
   return PassPlan{std::max<uint64_t>(P, 1), k, std::max(pix_chunk, 1u)};
 }
 
-// The render's plan: a fixed-spp render of two or more passes alternates them between two path
-// sets, each of at most half the budget (one set's kernels then fill the other's ramp-downs);
-// an adaptive render, whose passes depend on each other, and a one-pass render use one set of
-// the whole budget.
+// The render's plan.  A fixed-spp render of two or more passes alternates them between two path
+// sets, each of at most half the budget (one set's kernels then fill the other's ramp-downs); a
+// one-pass render uses one set of the whole budget.  An adaptive render keeps the whole budget
+// for the size of its passes (the samples per pixel per pass) and launches each pass's active
+// list in chunks that alternate between two sets of half the slots.
 struct RenderPlan {
-  PassPlan pass;
-  int sets;  // path sets of pass.paths_alloc slots each
+  PassPlan pass;      // pass.paths_alloc: slots of one set
+  int sets;           // path sets of pass.paths_alloc slots each
+  uint64_t pass_cap;  // adaptive: paths of one pass over both sets
 };
 inline RenderPlan plan_render_sets(uint32_t npix, uint32_t spp, int32_t samples_per_pass, bool adaptive,
                                    uint32_t target) {
-  if (adaptive) return RenderPlan{plan_render(npix, spp, samples_per_pass, true, target), 1};
+  if (adaptive) {
+    PassPlan p = plan_render(npix, spp, samples_per_pass, true, target);
+    const uint64_t cap = p.paths_alloc;
+    p.paths_alloc = std::max<uint64_t>(1, (cap + 1) / 2);
+    return RenderPlan{p, 2, cap};
+  }
   const PassPlan half = plan_render(npix, spp, samples_per_pass, false, std::max(target / 2, 1u));
   const uint64_t passes = (uint64_t)((npix + half.pix_chunk - 1) / half.pix_chunk) * ((spp + half.k - 1) / half.k);
-  if (passes >= 2) return RenderPlan{half, 2};
-  return RenderPlan{plan_render(npix, spp, samples_per_pass, false, target), 1};
+  if (passes >= 2) return RenderPlan{half, 2, half.paths_alloc};
+  const PassPlan one = plan_render(npix, spp, samples_per_pass, false, target);
+  return RenderPlan{one, 1, one.paths_alloc};
 }
 
 // One adaptive pass: n_active pixels are still sampling after `taken` samples each.  Every
 // active pixel gets *kk more samples (at least the rest of the min_spp phase, else up to 64 as
-// the slots allow); the active list is launched in chunks of *chunk pixels, with
-// chunk * kk <= P.  With more active pixels than slots, kk is 1 and the list is split.
-inline void plan_adaptive_pass(uint32_t n_active, uint32_t taken, uint32_t min_spp, uint32_t spp, uint64_t P,
-                               uint32_t* kk_out, uint32_t* chunk_out) {
+// the pass's cap allows); the active list is launched in chunks of *chunk pixels, with
+// chunk * kk <= set_cap (the slots of one path set).  A pass of at least 2^21 paths is split into
+// at least two chunks, so that both path sets work.
+inline void plan_adaptive_pass(uint32_t n_active, uint32_t taken, uint32_t min_spp, uint32_t spp, uint64_t pass_cap,
+                               uint64_t set_cap, uint32_t* kk_out, uint32_t* chunk_out) {
   n_active = std::max(n_active, 1u);
-  const uint64_t fill = std::max<uint64_t>(1, P / n_active);
+  const uint64_t fill = std::max<uint64_t>(1, pass_cap / n_active);
   uint32_t kk = taken < min_spp ? min_spp - taken : 1;
   kk = std::max<uint32_t>(kk, (uint32_t)std::min<uint64_t>(fill, 64));
   kk = std::min(kk, spp > taken ? spp - taken : 1u);
   kk = (uint32_t)std::min<uint64_t>(kk, fill);
+  kk = (uint32_t)std::min<uint64_t>(kk, std::max<uint64_t>(1, set_cap));
   kk = std::max(kk, 1u);
+  uint64_t chunk = std::min<uint64_t>(n_active, std::max<uint64_t>(1, set_cap / kk));
+  if (chunk == n_active && n_active >= 2 && (uint64_t)n_active * kk >= ((uint64_t)1 << 21)) chunk = (n_active + 1) / 2;
   *kk_out = kk;
-  *chunk_out = (uint32_t)std::min<uint64_t>(n_active, std::max<uint64_t>(1, P / kk));
+  *chunk_out = (uint32_t)chunk;
 }
 
 }  // namespace tpt
