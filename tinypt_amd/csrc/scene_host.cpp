@@ -257,6 +257,36 @@ struct SahBuilder {
 };
 }  // namespace
 
+// The FP32 triangle record of the vertices v[0..8] (see TriF32).
+TriF32 tri_f32(const double* v) {
+  double e1[3], e2[3], n[3];
+  for (int c = 0; c < 3; c++) {
+    e1[c] = v[3 + c] - v[c];
+    e2[c] = v[6 + c] - v[c];
+  }
+  auto cross = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  cross(e1, e2, n);
+  const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  double r0[3], r1[3];
+  cross(e2, n, r0);
+  cross(n, e1, r1);
+  TriF32 t{};
+  const bool ok = nn > 0.0 && std::isfinite(nn);
+  for (int c = 0; c < 3; c++) {
+    t.r0[c] = ok ? (float)(r0[c] / nn) : 0.f;
+    t.r1[c] = ok ? (float)(r1[c] / nn) : 0.f;
+    t.r2[c] = ok ? (float)(n[c] / nn) : 0.f;
+  }
+  t.r0[3] = (float)v[0];
+  t.r1[3] = (float)v[1];
+  t.r2[3] = (float)v[2];
+  return t;
+}
+
 RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf) {
   SahBuilder b(prim_bounds, max_leaf);
   b.out.indices.resize(prim_bounds.size());
@@ -604,13 +634,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
     for (size_t k = 0; k < f.indices.size(); k++) {
       int64_t t = b0 + f.indices[k];
       const double* v = &tri_v[9 * (size_t)t];
-      TriF32& o = tris32[(size_t)(b0 + (int64_t)k)];
-      for (int c = 0; c < 3; c++) {
-        o.v0[c] = (float)v[c];
-        o.e1[c] = (float)(v[3 + c] - v[c]);
-        o.e2[c] = (float)(v[6 + c] - v[c]);
-      }
-      o.v0[3] = o.e1[3] = o.e2[3] = 0.f;
+      tris32[(size_t)(b0 + (int64_t)k)] = tri_f32(v);
       tri_orig[(size_t)(b0 + (int64_t)k)] = (int32_t)f.indices[k];
       tri_mat32[(size_t)(b0 + (int64_t)k)] = tri_mat[(size_t)t];
     }
@@ -705,13 +729,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
       for (size_t k = 0; k < fr.indices.size(); k++) {
         const size_t t = (size_t)fr.indices[k];
         const double* v = &wv[9 * t];
-        TriF32& o = tris32[(size_t)base + k];
-        for (int c = 0; c < 3; c++) {
-          o.v0[c] = (float)v[c];
-          o.e1[c] = (float)(v[3 + c] - v[c]);
-          o.e2[c] = (float)(v[6 + c] - v[c]);
-        }
-        o.v0[3] = o.e1[3] = o.e2[3] = 0.f;
+        tris32[(size_t)base + k] = tri_f32(v);
         tri_orig[(size_t)base + k] = f_orig[t];
         tri_mat32[(size_t)base + k] = f_mat[t];
         tri_inst32[(size_t)base + k] = f_inst[t];

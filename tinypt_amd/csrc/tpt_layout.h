@@ -49,10 +49,15 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 is one 128 B fetch");
 
+// A triangle as the rows of the inverse of [e1 e2 n] (e1 = v1 - v0, e2 = v2 - v0, n = e1 x e2;
+// computed in FP64, rounded), each row's w one coordinate of v0: for a point q, the rows give
+// (u, v, w) of q - v0 in the triangle's own frame, so a ray hits the plane at w = 0 and the
+// triangle where u, v >= 0 and u + v <= 1.  Row 2 is n / |n|^2.  Degenerate triangles have zero
+// rows (never hit).
 struct alignas(16) TriF32 {
-  float v0[4];  // xyz, w unused
-  float e1[4];  // v1 - v0 (computed in FP64, rounded)
-  float e2[4];  // v2 - v0
+  float r0[4];  // (e2 x n) / |n|^2, v0.x
+  float r1[4];  // (n x e1) / |n|^2, v0.y
+  float r2[4];  // n / |n|^2, v0.z
 };
 
 struct alignas(16) InstF32 {

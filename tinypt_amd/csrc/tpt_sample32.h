@@ -177,15 +177,16 @@ __device__ __forceinline__ void hit_geometry32(const SceneF32& S, int32_t inst, 
     *ng = (*p - mk((float)sp[0], (float)sp[1], (float)sp[2])) / (float)sp[3];
     *mat = S.sph_mat[s];
   } else if ((uint32_t)inst & TPT_HIT_FLAT) {
-    // world-space triangle: e1 x e2, negated where the instance's A reverses orientation
+    // world-space triangle: e1 x e2 (the direction of the record's row 2), negated where the
+    // instance's A reverses orientation
     const float4* tp = reinterpret_cast<const float4*>(S.tris + prim);
-    f3 no = vnorm(cross(f3of(tp[1]), f3of(tp[2])));
+    f3 no = vnorm(f3of(tp[2]));
     *ng = (S.tri_inst[prim] & (int32_t)TPT_TRI_FLIP) ? -no : no;
     *mat = S.tri_mat[prim];
   } else {
     const InstF32& I = S.inst[inst];
     const float4* tp = reinterpret_cast<const float4*>(S.tris + prim);
-    f3 no = cross(f3of(tp[1]), f3of(tp[2]));
+    f3 no = f3of(tp[2]);  // along e1 x e2 (the record's row 2)
     const float4* nm = reinterpret_cast<const float4*>(I.nrm);
     float4 n0 = nm[0], n1 = nm[1], n2 = nm[2];
     *ng = vnorm(mk(n0.x * no.x + n0.y * no.y + n0.z * no.z, n1.x * no.x + n1.y * no.y + n1.z * no.z,

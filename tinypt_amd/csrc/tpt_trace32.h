@@ -321,21 +321,21 @@ struct TraceF32 {
         const uint32_t cnt = (((uint32_t)ref >> 28) & 7u) + 1u;
         for (uint32_t k = 0; k < cnt; k++) {
           const float4* tp = reinterpret_cast<const float4*>(S.tris + start + k);
-          const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
-          // Moller-Trumbore (geometry.rs:236-273)
-          const vec3<float> E1 = mk(e1.x, e1.y, e1.z), E2 = mk(e2.x, e2.y, e2.z);
-          const vec3<float> pvec = cross(cd, E2);
-          const float det = dot(E1, pvec);
-          if (fabsf(det) < det_eps) continue;
-          const float inv_det = 1.0f / det;
-          const vec3<float> tvec = mk(co.x - v0.x, co.y - v0.y, co.z - v0.z);
-          const float u = dot(tvec, pvec) * inv_det;
-          if (u < 0.f || u > 1.f) continue;
-          const vec3<float> qvec = cross(tvec, E1);
-          const float v = dot(cd, qvec) * inv_det;
-          if (v < 0.f || u + v > 1.f) continue;
-          const float th = dot(E2, qvec) * inv_det;
-          if (th < lo || th > best_t) continue;
+          const float4 a = tp[0], b = tp[1], c = tp[2];
+          // the ray in the triangle's frame (TriF32): the plane at w = 0, the triangle where
+          // u, v >= 0 and u + v <= 1 -- the hits of Triangle::hit (geometry.rs:236-273) up to
+          // rounding; comparisons written so that NaN (degenerate triangles, rays in the
+          // plane) never hits
+          const vec3<float> tv = mk(co.x - a.w, co.y - b.w, co.z - c.w);
+          const float dz = c.x * cd.x + c.y * cd.y + c.z * cd.z;
+          // the reference's |det| < 1e-10 test: det = -(n . d) = -dz |n|^2, |n|^2 = 1 / |row 2|^2
+          if (!(fabsf(dz) >= det_eps * (c.x * c.x + c.y * c.y + c.z * c.z))) continue;
+          const float th = -(c.x * tv.x + c.y * tv.y + c.z * tv.z) / dz;
+          if (!(th >= lo && th <= best_t)) continue;
+          const float u = fmaf(th, a.x * cd.x + a.y * cd.y + a.z * cd.z, a.x * tv.x + a.y * tv.y + a.z * tv.z);
+          if (!(u >= 0.f && u <= 1.f)) continue;
+          const float v = fmaf(th, b.x * cd.x + b.y * cd.y + b.z * cd.z, b.x * tv.x + b.y * tv.y + b.z * tv.z);
+          if (!(v >= 0.f && u + v <= 1.f)) continue;
           int32_t key = cur_key, hinst = cur_inst;
           if (cur_inst == S.flat_inst) {  // world-space triangle: its own instance decides ties
             const int32_t i0 = S.tri_inst[start + k] & (int32_t)~TPT_TRI_FLIP;
