@@ -569,6 +569,21 @@ struct EventTimer {
     }
     return hipEventRecord(pool->ev[used++], s);
   }
+  // Two events for a dispatch to record itself (its start and end), or nulls when timing is off.
+  hipError_t reserve_pair(hipEvent_t* a, hipEvent_t* b) {
+    *a = *b = nullptr;
+    if (!on) return hipSuccess;
+    while (pool->ev.size() < used + 2) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      pool->ev.push_back(e);
+    }
+    *a = pool->ev[used];
+    *b = pool->ev[used + 1];
+    used += 2;
+    return hipSuccess;
+  }
   double pairs_ms() {
     double t = 0;
     for (size_t i = 0; i + 1 < used; i += 2) {
@@ -589,9 +604,9 @@ int wf_bounces(PathSet* c, const WfParams& W, const PassArgs& A, EventTimer& tt,
   int cur = 0;
   for (int b = 0; b < W.max_bounces; b++) {
     const PassArgs* gen = b == 0 ? &A : nullptr;
-    HIPC(tt.mark());
-    HIPC(wf_launch_extend(W, cur, gen, P, s));
-    HIPC(tt.mark());
+    hipEvent_t e0, e1;
+    HIPC(tt.reserve_pair(&e0, &e1));
+    HIPC(wf_launch_extend(W, cur, gen, P, s, e0, e1));
     (*launches)++;
     HIPC(wf_launch_shade(W, cur, gen, P, s));
     if (W.lights.n > 0) {

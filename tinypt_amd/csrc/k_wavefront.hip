@@ -17,6 +17,8 @@
 #include "tpt_trace32.h"
 #include "wavefront.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <mutex>
 #include <unordered_map>
@@ -651,34 +653,43 @@ hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s
   hipLaunchKernelGGL(k_clear_radiance, dim3(grid_for(n)), dim3(TPT_WF_BLOCK), 0, s, W, n);
   return hipGetLastError();
 }
-template <bool OVF, bool GEN>
-static void launch_extend(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL((k_extend<OVF, GEN>), grid_of(max_n, (const void*)k_extend<OVF, GEN>), dim3(TPT_WF_BLOCK), 0, s, W,
-                     cur, A);
+// A launch, timed by events that the dispatch records itself when ev0 is given.
+template <class K>
+static void launch_timed(K kernel, dim3 grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const WfParams& W, int cur,
+                         const PassArgs& A) {
+  if (ev0) hipExtLaunchKernelGGL(kernel, grid, dim3(TPT_WF_BLOCK), 0, s, ev0, ev1, 0, W, cur, A);
+  else hipLaunchKernelGGL(kernel, grid, dim3(TPT_WF_BLOCK), 0, s, W, cur, A);
 }
 template <bool OVF, bool GEN>
-static void launch_extend_refill(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL((k_extend_refill<OVF, GEN>), grid_of(max_n, (const void*)k_extend_refill<OVF, GEN>),
-                     dim3(TPT_WF_BLOCK), 0, s, W, cur, A);
+static void launch_extend(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s, hipEvent_t ev0,
+                          hipEvent_t ev1) {
+  launch_timed(k_extend<OVF, GEN>, grid_of(max_n, (const void*)k_extend<OVF, GEN>), s, ev0, ev1, W, cur, A);
 }
-hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s) {
+template <bool OVF, bool GEN>
+static void launch_extend_refill(const WfParams& W, int cur, const PassArgs& A, uint32_t max_n, hipStream_t s,
+                                 hipEvent_t ev0, hipEvent_t ev1) {
+  launch_timed(k_extend_refill<OVF, GEN>, grid_of(max_n, (const void*)k_extend_refill<OVF, GEN>), s, ev0, ev1, W, cur,
+               A);
+}
+hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s,
+                            hipEvent_t ev0, hipEvent_t ev1) {
   const PassArgs A = gen ? *gen : PassArgs{};
   if (gen ? W.refill_first > 0 : W.refill > 0) {
     if (W.gstack) {
-      if (gen) launch_extend_refill<true, true>(W, cur, A, max_n, s);
-      else launch_extend_refill<true, false>(W, cur, A, max_n, s);
+      if (gen) launch_extend_refill<true, true>(W, cur, A, max_n, s, ev0, ev1);
+      else launch_extend_refill<true, false>(W, cur, A, max_n, s, ev0, ev1);
     } else {
-      if (gen) launch_extend_refill<false, true>(W, cur, A, max_n, s);
-      else launch_extend_refill<false, false>(W, cur, A, max_n, s);
+      if (gen) launch_extend_refill<false, true>(W, cur, A, max_n, s, ev0, ev1);
+      else launch_extend_refill<false, false>(W, cur, A, max_n, s, ev0, ev1);
     }
     return hipGetLastError();
   }
   if (W.gstack) {
-    if (gen) launch_extend<true, true>(W, cur, A, max_n, s);
-    else launch_extend<true, false>(W, cur, A, max_n, s);
+    if (gen) launch_extend<true, true>(W, cur, A, max_n, s, ev0, ev1);
+    else launch_extend<true, false>(W, cur, A, max_n, s, ev0, ev1);
   } else {
-    if (gen) launch_extend<false, true>(W, cur, A, max_n, s);
-    else launch_extend<false, false>(W, cur, A, max_n, s);
+    if (gen) launch_extend<false, true>(W, cur, A, max_n, s, ev0, ev1);
+    else launch_extend<false, false>(W, cur, A, max_n, s, ev0, ev1);
   }
   return hipGetLastError();
 }

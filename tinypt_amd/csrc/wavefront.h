@@ -161,7 +161,10 @@ struct AdaptiveArgs {
 
 hipError_t wf_launch_clear_radiance(const WfParams& W, uint32_t n, hipStream_t s);
 // gen: the pass's paths, for the first bounce (its kernels generate the camera rays); null after
-hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
+// ev0 / ev1 (optional): events the dispatch itself records at the kernel's start and end (no
+// separate marker packets between kernels)
+hipError_t wf_launch_extend(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s,
+                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t wf_launch_shade(const WfParams& W, int cur, const PassArgs* gen, uint32_t max_n, hipStream_t s);
 // cur: the queue of paths shaded this bounce; gen: the pass's paths on the first bounce (their
 // rays are generated again from their ids), null after
