@@ -240,8 +240,8 @@ def main():
                     help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the renderer has no multi-GPU path (how it would shard is "
-              "described with the design notes); run it with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the renderer has no multi-GPU path; run it with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import tinypt_amd as T
