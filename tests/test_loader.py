@@ -234,6 +234,15 @@ def test_errors(tmp_path):
             load_scene(str(p), _cfg())
     with pytest.raises(T.TptError):
         load_obj_triangles(str(tmp_path / "missing.obj"))
+    # no <film> and a zero film size in the configuration: the camera would get a 0/0 aspect ratio
+    p = tmp_path / "nofilm.xml"
+    p.write_text("<scene><sensor type='perspective'/></scene>")
+    c = _cfg()
+    c.width = 0
+    with pytest.raises(T.TptError, match="film width and height"):
+        load_scene(str(p), c)
+    p.write_text("<scene><film type='hdrfilm'><integer name='width' value='8'/></film></scene>")
+    assert load_scene(str(p), c).cam.aspect == 8 / 9  # <film> supplies the missing width
 
 
 # ---------------------------------------------------------------- the sample scenes

@@ -876,6 +876,9 @@ int tpt_scene_load_xml(const char* path, tpt_render_config* cfg, tpt_loaded_scen
         if (el_int(c, "rr_depth", &v)) cfg->rr_start = (int32_t)std::min<uint64_t>(v, INT32_MAX);
       }
     }
+    if (cfg->width <= 0 || cfg->height <= 0)  // the camera's aspect ratio comes from the film size
+      return set_last_error(TPT_ERR_INVALID_ARGUMENT,
+                            "film width and height must be positive: set them in cfg or in <film>");
     const std::string base = dir_of(path);
     const double aspect = (double)cfg->width / (double)cfg->height;
     bool have_cam = false, have_env = false;
