@@ -31,8 +31,10 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-
 # forward kernels and the adjoint's replay of the same code round identically (a path replayed
 # for the gradient is then the path the image summed).  FP32 division and square root use the
 # hardware reciprocal / square-root approximations instead of correctly rounded sequences
-# (a few ulp; the FP64 kernels are unaffected).
-FP32_FLAGS = ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+# (a few ulp; the FP64 kernels are unaffected).  FP32 subnormals flush to zero, so that those
+# approximations are the bare instructions, without the exponent scaling that keeps subnormal
+# operands exact; FP64 keeps its subnormals.
+FP32_FLAGS = ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
 SOURCES = [
     ("api.hip", HIP_FLAGS),
     ("k_trace.hip", HIP_FLAGS + FP32_FLAGS),
