@@ -99,13 +99,15 @@ TPT_HD bool sphere_hit64(const double* s, vec3<float> of, vec3<float> df, double
   double r = s[3];
   vec3<double> f = o - c;
   double a = dot(d, d);
+  double ia = 1.0 / a;  // d is a unit FP32 vector: a is 1 within a few ulp, and x * ia is x / a
+                        // within an FP64 ulp, far below the FP32 rounding of the root
   double bp = -dot(f, d);
-  vec3<double> l = f + (bp / a) * d;
+  vec3<double> l = f + (bp * ia) * d;
   double disc = a * (r * r - dot(l, l));  // = b^2 - a c without cancellation
   if (disc <= 0.0) return false;
   double cc = dot(f, f) - r * r;
   double q = bp + copysign(sqrt(disc), bp);
-  double ta = cc / q, tb = q / a;
+  double ta = cc / q, tb = q * ia;
   double tn = fmin(ta, tb), tf = fmax(ta, tb);
   double t = tn;
   if (t < tmin || t > tmax) {
