@@ -138,7 +138,7 @@ __device__ __forceinline__ bool bsdf_sample(int kind, vec3<T> c, T s, vec3<T> d_
     }
     case TPT_MAT_DIELECTRIC: {
       T ior = s;
-      T eta = entering ? T(1) / ior : ior;
+      T eta = entering ? div_approx(T(1), ior) : ior;
       T cos_i = fmax(dot(-d_in, n), T(0));
       T r0 = (ior - T(1)) / (ior + T(1));
       T f0 = r0 * r0;
@@ -241,7 +241,7 @@ __device__ __forceinline__ bool mat_is_delta(int kind) { return kind == TPT_MAT_
 
 template <class T> __device__ __forceinline__ T mis_weight(T a, T b) {  // integrator.rs:221-225
   T a2 = a * a, b2 = b * b;
-  return a2 + b2 > T(0) ? a2 / (a2 + b2) : T(0);
+  return a2 + b2 > T(0) ? div_approx(a2, a2 + b2) : T(0);
 }
 
 // cdf_search (world.rs:324-336, env.rs:185-197): largest lo with cdf[lo] <= x, starting at 0

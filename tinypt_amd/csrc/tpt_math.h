@@ -24,6 +24,23 @@ template <class T> TPT_HD vec3<T> operator-(vec3<T> a, vec3<T> b) { return mk(a.
 template <class T> TPT_HD vec3<T> operator*(vec3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
 template <class T> TPT_HD vec3<T> operator*(T s, vec3<T> a) { return mk(a.x * s, a.y * s, a.z * s); }
 template <class T> TPT_HD vec3<T> operator/(vec3<T> a, T s) { return mk(a.x / s, a.y / s, a.z / s); }
+// FP32 division as the FP32 kernels compute it on the device: the hardware reciprocal (about
+// 1 ulp) times the numerator, which is what their approximate-division build gives a / b.  Written
+// out because the compiler drops that approximation from a division it rewrites (folding a
+// negation or a select into it), leaving the correctly rounded dozen-instruction sequence there.
+// The host build and FP64 divide exactly.
+TPT_HD float div_approx(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return a * __builtin_amdgcn_rcpf(b);
+#else
+  return a / b;
+#endif
+}
+TPT_HD double div_approx(double a, double b) { return a / b; }
+TPT_HD vec3<float> operator/(vec3<float> a, float s) {
+  const float r = div_approx(1.f, s);
+  return mk(a.x * r, a.y * r, a.z * r);
+}
 template <class T> TPT_HD vec3<T> operator-(vec3<T> a) { return mk(-a.x, -a.y, -a.z); }
 template <class T> TPT_HD T dot(vec3<T> a, vec3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class T> TPT_HD vec3<T> cross(vec3<T> a, vec3<T> b) {
