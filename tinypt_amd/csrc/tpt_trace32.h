@@ -73,6 +73,9 @@ struct LaneStackT {
   }
 };
 using LaneStack = LaneStackT<true>;
+// Whether a traversal stack type carries the overflow path (deep trees); true unless known.
+template <class Stack> struct StackOverflows { static constexpr bool value = true; };
+template <> struct StackOverflows<LaneStackT<false>> { static constexpr bool value = false; };
 
 TPT_HD float safe_rcp(float d) { return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d)); }
 
@@ -91,9 +94,26 @@ TPT_HD double approx_rcp64(double x) {
 #define TPT_KEY_INST(i) (0x3fffffff - (int32_t)(i))
 #define TPT_STACK_SENTINEL TPT_REF_EMPTY
 
+// An empty asm statement that makes v opaque to the optimiser at this point: FP64 terms that
+// depend only on the ray (its direction's squared length and reciprocal) would otherwise be
+// hoisted out of the traversal loop from the sphere test and the instance entry, and hold FP64
+// registers -- spilled to scratch -- on every ray, whether it meets a sphere or instance or not.
+// No instruction is emitted.  The sphere test always keeps its terms; the instance entry keeps
+// them in the kernels for trees within the LDS stack (scenes of few instances or one flattened
+// tree, entered by few rays or none), while the deep-tree kernels, whose rays enter instances
+// at most steps, are faster with them hoisted.
+TPT_HD void opaque(float& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v));
+#endif
+}
+
 // FP64 sphere root in [tmin, tmax] (near root first, then far), geometry.rs:174-195.
 TPT_HD bool sphere_hit64(const double* s, vec3<float> of, vec3<float> df, double tmin, double tmax,
                                              double* t_out) {
+  opaque(df.x);  // the direction's FP64 terms stay in this test (see opaque)
+  opaque(df.y);
+  opaque(df.z);
   vec3<double> o = mk((double)of.x, (double)of.y, (double)of.z), d = mk((double)df.x, (double)df.y, (double)df.z);
   vec3<double> c = mk(s[0], s[1], s[2]);
   double r = s[3];
@@ -292,7 +312,11 @@ struct TraceF32 {
             {
               const double* c = S.inst_c64 + 3 * (size_t)item;
               const InstF64& J = S.inst64[item];
-              const double dx = d.x, dy = d.y, dz = d.z, ox = o.x, oy = o.y, oz = o.z;
+              float fdx = d.x, fdy = d.y, fdz = d.z, fox = o.x, foy = o.y, foz = o.z;
+              if constexpr (!StackOverflows<Stack>::value) {
+                opaque(fdx); opaque(fdy); opaque(fdz); opaque(fox); opaque(foy); opaque(foz);
+              }
+              const double dx = fdx, dy = fdy, dz = fdz, ox = fox, oy = foy, oz = foz;
               shift = (float)(((c[0] - ox) * dx + (c[1] - oy) * dy + (c[2] - oz) * dz) *
                               approx_rcp64(dx * dx + dy * dy + dz * dz));
               const double s = shift;
