@@ -139,9 +139,13 @@ k_extend(WfParams W, int cur, PassArgs A) {
 // cursor; dynamic ray fetch, Aila and Laine 2009).  The hits are those of k_extend: each ray's traversal is the same sequence of steps.
 // The cursor (TPT_CNT_FETCH) is zero at launch: the render clears it, and every k_shade launch
 // clears it for the next extension launch.  Every wave exits once the cursor has passed the
-// queue's end and its lanes are idle.
+// queue's end and its lanes are idle.  With the overflow path the resumable state needs more
+// registers: five waves per SIMD (96 VGPRs) for the later bounces and four (117) for the first
+// keep it out of scratch, where six and five spilled 44 and 64 bytes per lane that the loop
+// reloaded at every step (measured: the 1.31M-triangle mesh 4% and the 4096 instances 11%
+// faster).
 template <bool OVF, bool GEN>
-__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(GEN ? 5 : 6)))
+__global__ void __launch_bounds__(TPT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(GEN && OVF ? 4 : GEN || OVF ? 5 : 6)))
 k_extend_refill(WfParams W, int cur, PassArgs A) {
   __shared__ int32_t s_stack[TPT_WF_STACK * TPT_WF_BLOCK];
   QueueMap qm;
