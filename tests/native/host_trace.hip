@@ -143,3 +143,16 @@ extern "C" int tpt_test_selfocc(const tpt_scene_desc* d, int64_t n, const float*
   }
   return 0;
 }
+
+// The self-occlusion test's per-component quotient (s64::div3: one reciprocal, a residual
+// correction per component) for n vectors v[3i..] and divisors l[i], written to out[3i..], to be
+// compared with plain division.
+extern "C" int tpt_test_div3(int64_t n, const double* v, const double* l, double* out) {
+  for (int64_t i = 0; i < n; i++) {
+    const D3 q = s64::div3(D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]}, l[i]);
+    out[3 * i] = q.x;
+    out[3 * i + 1] = q.y;
+    out[3 * i + 2] = q.z;
+  }
+  return 0;
+}

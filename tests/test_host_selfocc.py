@@ -98,3 +98,30 @@ def test_self_occlusion_matches_the_oracle(name, make, n):
           f"z {(r_prod - r_ref) / se:+.2f}")
     assert 0.05 < r_ref < 0.95
     assert abs(r_prod - r_ref) < 4 * se, (r_prod, r_ref, se)
+
+
+def test_shared_reciprocal_quotient_equals_division():
+    """The decision normalises directions with one correctly rounded reciprocal and a residual
+    correction per component (Markstein's theorem); its quotients must be those of the
+    division, bit for bit, signed zeros included."""
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    v = rng.normal(size=(n, 3)) * 10.0 ** rng.uniform(-6, 6, size=(n, 1))
+    v32 = v.astype(np.float32).astype(np.float64)  # FP32-derived directions, as the kernels' rays
+    v = np.concatenate([v, v32])
+    l = np.sqrt((v * v).sum(axis=1))
+    other = 10.0 ** rng.uniform(-3, 3, size=len(v))  # divisors unrelated to the vector
+    v = np.concatenate([v, v])
+    l = np.concatenate([l, other])
+    zeros = np.zeros((8, 3))
+    zeros[1::2] = -0.0
+    zeros[2:4, 0] = 0.5
+    v = np.concatenate([v, zeros])
+    l = np.concatenate([l, np.full(8, 1.7)])
+    out = np.empty_like(v)
+    rc = _lib().tpt_test_div3(C.c_int64(len(l)), np.ascontiguousarray(v).ctypes.data_as(C.POINTER(C.c_double)),
+                              np.ascontiguousarray(l).ctypes.data_as(C.POINTER(C.c_double)),
+                              out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    want = v / l[:, None]
+    assert np.array_equal(out.view(np.uint64), want.view(np.uint64))

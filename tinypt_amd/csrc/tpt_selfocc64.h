@@ -47,6 +47,22 @@ TPT_HD D3 mv(const double* m, D3 v) {  // Mat3::mul_vec, row-major
   return D3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z};
 }
 TPT_HD D3 ld(const double* p) { return D3{p[0], p[1], p[2]}; }
+// v / l per component, each quotient correctly rounded as by a division, from one correctly
+// rounded reciprocal y = 1 / l: q = v.x * y lies within an ulp of v.x / l, the residual
+// v.x - q l is exact under an FMA, and q + residual * y rounded once is v.x / l rounded to
+// nearest (Markstein's theorem: P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Develop. 34(1), 1990).  Valid while no quotient or
+// residual underflows, which holds for these direction components (|v.x| <= l, and l far above
+// the subnormal range); one division instead of three.  The residual is formed as q l - x and
+// subtracted so that a zero component keeps its sign, as the division's quotient does.
+TPT_HD double div_by(double x, double l, double y) {
+  const double q = x * y;
+  return fma(-fma(q, l, -x), y, q);
+}
+TPT_HD D3 div3(D3 v, double l) {
+  const double y = 1.0 / l;
+  return D3{div_by(v.x, l, y), div_by(v.y, l, y), div_by(v.z, l, y)};
+}
 }  // namespace s64
 
 // World::hit on triangle `tri` (mesh-local, original order) of instance `inst`
@@ -63,7 +79,7 @@ TPT_HD bool ref_instance_tri64(const SceneF64& S, int inst, int tri, D3 o, D3 d,
   D3 o_obj = mv(I.a_inv, sub(o, t));
   D3 d_obj = mv(I.a_inv, d);
   double l = fmax(sqrt(dot(d_obj, d_obj)), 1e-30);
-  d_obj = D3{d_obj.x / l, d_obj.y / l, d_obj.z / l};
+  d_obj = div3(d_obj, l);
   D3 e1 = sub(v1, v0), e2 = sub(v2, v0);
   D3 pvec = cross(d_obj, e2);
   double det = dot(e1, pvec);
@@ -148,7 +164,7 @@ TPT_HD bool light_self_occluded64(const SceneF64& S, const LightsDev& Ls, int id
     D3 tl = sub(pos, hit_p);
     double dist2 = dot(tl, tl);
     double dist = sqrt(dist2);
-    D3 wi = D3{tl.x / dist, tl.y / dist, tl.z / dist};
+    D3 wi = div3(tl, dist);
     D3 o = add(hit_p, mul(wi, eps));
     double tmax = fmax(dist - eps, eps);
     double t;
@@ -167,7 +183,7 @@ TPT_HD bool light_self_occluded64(const SceneF64& S, const LightsDev& Ls, int id
   D3 tl = sub(pos, hit_p);
   double dist2 = dot(tl, tl);
   double dist = sqrt(dist2);
-  D3 wi = D3{tl.x / dist, tl.y / dist, tl.z / dist};
+  D3 wi = div3(tl, dist);
   D3 o = add(hit_p, mul(wi, eps));
   double tmax = fmax(dist - eps, eps);
   // World::hit for this instance (world.rs:92-108), Triangle::hit on the sampled triangle
