@@ -99,3 +99,19 @@ def test_host_only_scene_cannot_be_rendered():
 def test_context_creation_fails_cleanly_without_a_device():
     with pytest.raises(T.TptError):
         T.Context(0)
+
+
+def test_c_client_compiles_and_links(tmp_path):
+    """The plain C client of the header (tests/native/c_abi_render.c, which the GPU tests run)
+    builds as C with warnings as errors and links against the library: the header is valid C and
+    every call it makes resolves.  It is not run here (it needs a device)."""
+    import shutil
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    libdir = os.path.join(ROOT, "tinypt_amd", "lib")
+    exe = str(tmp_path / "c_abi_render")
+    subprocess.run([gcc, "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-o", exe,
+                    os.path.join(ROOT, "tests", "native", "c_abi_render.c"), f"-L{libdir}", "-ltinypt_gpu",
+                    f"-Wl,-rpath,{libdir}", "-Wl,--no-undefined"], check=True)
+    assert os.path.getsize(exe) > 0
