@@ -335,7 +335,9 @@ int tpt_loaded_scene_destroy(tpt_loaded_scene* scene);
 
 /* load_obj_triangles (obj_loader.rs:101-109): triangles of a Wavefront OBJ file (v and f
  * records; polygons fan-triangulated; negative indices count from the end; other records
- * ignored).  tri_vertices[n*9] (v0, v1, v2); pass NULL to query *n_tris only. */
+ * ignored).  tri_vertices[n*9] (v0, v1, v2); pass NULL to query *n_tris only.  With a buffer,
+ * *n_tris on entry is its capacity in triangles (TPT_ERR_INVALID_ARGUMENT, nothing written, if
+ * the file holds more) and on return the number of triangles. */
 int tpt_obj_load(const char* path, int64_t* n_tris, double* tri_vertices);
 
 #ifdef __cplusplus

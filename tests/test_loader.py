@@ -7,6 +7,7 @@ sample/*.xml and cube.obj without comments) are then loaded and checked against 
 from the same parameters in Python, transform by transform, and the loaded Cornell box's
 BVHs and light table against the oracle's.
 """
+import ctypes as C
 import math
 import os
 import struct
@@ -234,6 +235,17 @@ def test_errors(tmp_path):
             load_scene(str(p), _cfg())
     with pytest.raises(T.TptError):
         load_obj_triangles(str(tmp_path / "missing.obj"))
+    # a buffer smaller than the file's triangle count is refused, not overrun
+    p = tmp_path / "quad.obj"
+    p.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n")
+    lib = T.load_library()
+    n = C.c_int64(1)
+    buf = np.full(3 * 9, 7.0)
+    rc = lib.tpt_obj_load(str(p).encode(), C.byref(n), buf.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc != 0 and n.value == 2 and (buf == 7.0).all()
+    n = C.c_int64(3)
+    assert lib.tpt_obj_load(str(p).encode(), C.byref(n), buf.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    assert n.value == 2 and (buf[18:] == 7.0).all() and not (buf[:18] == 7.0).all()
     # no <film> and a zero film size in the configuration: the camera would get a 0/0 aspect ratio
     p = tmp_path / "nofilm.xml"
     p.write_text("<scene><sensor type='perspective'/></scene>")

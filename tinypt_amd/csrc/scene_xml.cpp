@@ -979,8 +979,12 @@ int tpt_obj_load(const char* path, int64_t* n_tris, double* tri_vertices) {
     std::vector<double> tris;
     std::string err;
     if (!load_obj(path, &tris, &err)) return set_last_error(TPT_ERR_INVALID_ARGUMENT, err);
-    *n_tris = (int64_t)(tris.size() / 9);
-    if (tri_vertices) std::memcpy(tri_vertices, tris.data(), sizeof(double) * tris.size());
+    const int64_t n = (int64_t)(tris.size() / 9), cap = *n_tris;
+    *n_tris = n;
+    if (tri_vertices && n > cap)
+      return set_last_error(TPT_ERR_INVALID_ARGUMENT, "the buffer holds " + std::to_string(cap) +
+                                                          " triangles, the file " + std::to_string(n));
+    if (tri_vertices && n > 0) std::memcpy(tri_vertices, tris.data(), sizeof(double) * tris.size());
     return TPT_OK;
   } catch (const std::bad_alloc&) {
     return set_last_error(TPT_ERR_OUT_OF_MEMORY, "host allocation failed while reading the OBJ file");
