@@ -12,7 +12,10 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(os.path.dirname(HERE)), "tinypt_amd", "csrc")
-LIB = os.path.join(HERE, "libhost_trace.so")
+# TPT_HOST_TRACE_LIB names a library built elsewhere (the sanitizer test's instrumented build),
+# used as it is.
+LIB_OVERRIDE = os.environ.get("TPT_HOST_TRACE_LIB")
+LIB = LIB_OVERRIDE or os.path.join(HERE, "libhost_trace.so")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("TPT_OFFLOAD_ARCH", "gfx950")
 
@@ -54,6 +57,8 @@ def build(verbose: bool = False) -> str:
 
 def is_stale() -> bool:
     """True when the library is missing or older than a source it is built from."""
+    if LIB_OVERRIDE:
+        return False
     if not os.path.exists(LIB):
         return True
     t = os.path.getmtime(LIB)
