@@ -33,7 +33,7 @@ def build(verbose: bool = False) -> str:
         scene_obj = os.path.join(HERE, f"scene_host.{tag}.o")
         hook_obj = os.path.join(HERE, f"host_trace.{tag}.o")
         cmds = [
-            ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC, "-c",
+            ["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC, "-c",
              os.path.join(CSRC, "scene_host.cpp"), "-o", scene_obj],
             [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-function", "-Xclang",
              "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-c", os.path.join(HERE, "host_trace.hip"),

@@ -42,7 +42,7 @@ SOURCES = [
     ("k_adjoint.hip", HIP_FLAGS + FP32_FLAGS),
     ("k_plain64.hip", HIP_FLAGS + ["-ffp-contract=off"]),
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math", "-I", CSRC]
 HOST_SOURCES = [
     ("scene_host.cpp", HOST_FLAGS),
     ("scene_xml.cpp", HOST_FLAGS),

@@ -2,14 +2,53 @@
 #include "scene_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <limits>
+#include <mutex>
 #include <numeric>
+#include <system_error>
+#include <thread>
 
 namespace tpt {
+
+// Runs the tasks on up to 16 host threads (the calling thread among them), largest first as
+// given; the first exception a task throws is rethrown here once every task has finished.  A
+// thread that cannot be started leaves its share to the others.
+static void run_parallel(const std::vector<std::function<void()>>& tasks) {
+  const size_t n = tasks.size();
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(n, std::min(hw, 16u));
+  std::atomic<size_t> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  auto work = [&] {
+    for (size_t i; (i = next.fetch_add(1)) < n;) {
+      try {
+        tasks[i]();
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!err) err = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; t++) {
+    try {
+      pool.emplace_back(work);
+    } catch (const std::system_error&) {
+      break;
+    }
+  }
+  work();
+  for (std::thread& th : pool) th.join();
+  if (err) std::rethrow_exception(err);
+}
 
 static const double kPi = 3.14159265358979323846;
 static const double kInf = std::numeric_limits<double>::infinity();
@@ -55,16 +94,27 @@ DAabb triangle_bounds(const double* v) {
 // arithmetic is the reference's f64 arithmetic so the node array comes out identical.
 namespace {
 struct RefBuilder {
-  const std::vector<DAabb>& bounds;
-  std::vector<double> cent;  // 3 per primitive
+  // A primitive's box, centroid and index in the order of out.indices: the builder moves these
+  // records, so each pass over a node's range reads memory in sequence.
+  struct Prim {
+    DAabb box;
+    double c[3];
+    int64_t idx;
+  };
+  std::vector<Prim> rec;
+  std::vector<Prim> tmp;         // scatter target of the bin sort
+  std::vector<uint8_t> bin_buf;  // bins of the current range
   size_t leaf_size;
   RefBvh out;
   static const size_t BINS = 8;
 
-  RefBuilder(const std::vector<DAabb>& b, size_t ls) : bounds(b), leaf_size(ls) {
-    cent.resize(3 * b.size());
-    for (size_t i = 0; i < b.size(); i++)
-      for (int k = 0; k < 3; k++) cent[3 * i + k] = (b[i].mn[k] + b[i].mx[k]) * 0.5;  // Aabb::centroid
+  RefBuilder(const std::vector<DAabb>& b, size_t ls) : leaf_size(ls) {
+    rec.resize(b.size());
+    for (size_t i = 0; i < b.size(); i++) {
+      rec[i].box = b[i];
+      for (int k = 0; k < 3; k++) rec[i].c[k] = (b[i].mn[k] + b[i].mx[k]) * 0.5;  // Aabb::centroid
+      rec[i].idx = (int64_t)i;
+    }
   }
 
   int32_t node(size_t start, size_t end, int depth) {
@@ -72,9 +122,8 @@ struct RefBuilder {
     size_t n = end - start;
     DAabb bbox = empty_box(), cbox = empty_box();
     for (size_t k = start; k < end; k++) {
-      size_t i = (size_t)out.indices[k];
-      bbox = unite(bbox, bounds[i]);
-      cbox = grow(cbox, &cent[3 * i]);
+      bbox = unite(bbox, rec[k].box);
+      cbox = grow(cbox, rec[k].c);
     }
     int32_t idx = (int32_t)out.nodes.size();
     out.nodes.push_back(RefNode{bbox, -1, -1, 0, 0});
@@ -91,16 +140,18 @@ struct RefBuilder {
       double extent = maxc - minc;
       if (extent > 1e-12) {
         double inv_extent = 1.0 / extent;
-        auto bin_of = [&](int64_t prim) {
-          size_t b = sat_usize((cent[3 * (size_t)prim + axis] - minc) * inv_extent * (double)BINS);
+        auto bin_of = [&](const Prim& q) {
+          size_t b = sat_usize((q.c[axis] - minc) * inv_extent * (double)BINS);
           return b >= BINS ? BINS - 1 : b;
         };
         DAabb bb[BINS];
         size_t bc[BINS];
         for (size_t b = 0; b < BINS; b++) { bb[b] = empty_box(); bc[b] = 0; }
+        bin_buf.resize(n);
         for (size_t k = start; k < end; k++) {
-          size_t b = bin_of(out.indices[k]);
-          bb[b] = unite(bb[b], bounds[(size_t)out.indices[k]]);
+          const size_t b = bin_of(rec[k]);
+          bin_buf[k - start] = (uint8_t)b;
+          bb[b] = unite(bb[b], rec[k].box);
           bc[b] += 1;
         }
         DAabb lb[BINS], rb[BINS];
@@ -132,8 +183,13 @@ struct RefBuilder {
           if (cost < best_cost) { best_cost = cost; best_split = b + 1; }
         }
         if (std::isfinite(best_cost)) {
-          std::stable_sort(out.indices.begin() + (std::ptrdiff_t)start, out.indices.begin() + (std::ptrdiff_t)end,
-                           [&](int64_t a, int64_t b) { return bin_of(a) < bin_of(b); });
+          // the reference's stable sort of the range by bin, as a stable counting sort (the
+          // order of a stable sort by a key is the one order that keeps equal keys in place)
+          size_t off[BINS];
+          for (size_t b = 0, o = 0; b < BINS; b++) { off[b] = o; o += bc[b]; }
+          tmp.resize(n);
+          for (size_t k = start; k < end; k++) tmp[off[bin_buf[k - start]]++] = rec[k];
+          std::copy(tmp.begin(), tmp.begin() + (std::ptrdiff_t)n, rec.begin() + (std::ptrdiff_t)start);
           size_t left_total = 0;
           for (size_t b = 0; b < best_split; b++) left_total += bc[b];
           if (left_total > 0 && left_total < n) mid = start + left_total;
@@ -151,9 +207,9 @@ struct RefBuilder {
 
 RefBvh build_ref_bvh(const std::vector<DAabb>& prim_bounds, size_t leaf_size) {
   RefBuilder b(prim_bounds, leaf_size);
-  b.out.indices.resize(prim_bounds.size());
-  std::iota(b.out.indices.begin(), b.out.indices.end(), 0);
   if (!prim_bounds.empty()) b.node(0, prim_bounds.size(), 1);
+  b.out.indices.resize(prim_bounds.size());
+  for (size_t i = 0; i < b.rec.size(); i++) b.out.indices[i] = b.rec[i].idx;
   return std::move(b.out);
 }
 
@@ -168,64 +224,87 @@ struct SahBuilder {
   static constexpr int kBins = 32;
   static constexpr double kTraverse = 1.0;   // cost of one binary node visit
   static constexpr double kIntersect = 1.0;  // cost of one triangle test
-  const std::vector<DAabb>& pb;
-  std::vector<double> cen;  // 3 per primitive: box centres
+  // A primitive's box, box centre and index, kept in the order of the node ranges: the builder
+  // partitions these records themselves, so every pass over a node's range reads memory in
+  // sequence rather than through a permuted index list.
+  struct Prim {
+    DAabb box;
+    double c[3];
+    int64_t idx;
+  };
+  std::vector<Prim> rec;
   size_t max_leaf;
   RefBvh out;
-  SahBuilder(const std::vector<DAabb>& b, size_t leaf) : pb(b), max_leaf(leaf) {
-    cen.resize(3 * b.size());
-    for (size_t i = 0; i < b.size(); i++)
-      for (int k = 0; k < 3; k++) cen[3 * i + k] = 0.5 * (b[i].mn[k] + b[i].mx[k]);
+  // Subtrees whose root lies at most this deep are built on threads of their own, each into its
+  // own node array, and spliced in pre-order: the node array is the serial build's.
+  static constexpr int kParallelDepth = 3;
+  static constexpr size_t kParallelMin = 1u << 15;  // primitives worth a thread
+  SahBuilder(const std::vector<DAabb>& b, size_t leaf) : max_leaf(leaf) {
+    rec.resize(b.size());
+    for (size_t i = 0; i < b.size(); i++) {
+      rec[i].box = b[i];
+      for (int k = 0; k < 3; k++) rec[i].c[k] = 0.5 * (b[i].mn[k] + b[i].mx[k]);
+      rec[i].idx = (int64_t)i;
+    }
   }
   static double half_area(const DAabb& b) {
     const double x = b.mx[0] - b.mn[0], y = b.mx[1] - b.mn[1], z = b.mx[2] - b.mn[2];
     return x * y + y * z + z * x;
   }
-  int32_t node(size_t lo, size_t hi, int depth) {
-    out.depth = std::max(out.depth, depth);
+  // Builds the subtree of rec[lo, hi) into `nodes` (pre-order, child indices into `nodes`);
+  // `right` is the partition's scratch, `max_depth` the deepest level reached.
+  int32_t node(size_t lo, size_t hi, int depth, std::vector<RefNode>& nodes, std::vector<Prim>& right, int& max_depth) {
+    max_depth = std::max(max_depth, depth);
     DAabb box = empty_box(), cb = empty_box();
     for (size_t i = lo; i < hi; i++) {
-      const size_t p = (size_t)out.indices[i];
-      box = unite(box, pb[p]);
-      cb = grow(cb, &cen[3 * p]);
+      box = unite(box, rec[i].box);
+      cb = grow(cb, rec[i].c);
     }
-    const int32_t me = (int32_t)out.nodes.size();
-    out.nodes.push_back(RefNode{box, -1, -1, (uint32_t)lo, (uint32_t)(hi - lo)});
+    const int32_t me = (int32_t)nodes.size();
+    nodes.push_back(RefNode{box, -1, -1, (uint32_t)lo, (uint32_t)(hi - lo)});
     const size_t n = hi - lo;
     if (n <= 1) return me;
-    // best binned split over the three axes
+    // binned SAH over the three axes, the bins of all three filled in one pass
+    DAabb bb[3][kBins];
+    size_t bc[3][kBins];
+    double scale[3];
+    bool use[3];
+    for (int axis = 0; axis < 3; axis++) {
+      const double ext = cb.mx[axis] - cb.mn[axis];
+      use[axis] = ext > 0.0;
+      scale[axis] = use[axis] ? kBins / ext : 0.0;
+      for (int b = 0; b < kBins; b++) { bb[axis][b] = empty_box(); bc[axis][b] = 0; }
+    }
+    for (size_t i = lo; i < hi; i++) {
+      const Prim& q = rec[i];
+      for (int axis = 0; axis < 3; axis++) {
+        if (!use[axis]) continue;
+        int b = (int)((q.c[axis] - cb.mn[axis]) * scale[axis]);
+        b = std::min(std::max(b, 0), kBins - 1);
+        bb[axis][b] = unite(bb[axis][b], q.box);
+        bc[axis][b]++;
+      }
+    }
     double best = kInf;
     int best_axis = -1, best_bin = 0;
     const double parent = half_area(box);
     for (int axis = 0; axis < 3; axis++) {
-      const double ext = cb.mx[axis] - cb.mn[axis];
-      if (!(ext > 0.0)) continue;
-      DAabb bb[kBins];
-      size_t bc[kBins];
-      for (int b = 0; b < kBins; b++) { bb[b] = empty_box(); bc[b] = 0; }
-      const double scale = kBins / ext;
-      for (size_t i = lo; i < hi; i++) {
-        const size_t p = (size_t)out.indices[i];
-        int b = (int)((cen[3 * p + axis] - cb.mn[axis]) * scale);
-        b = std::min(std::max(b, 0), kBins - 1);
-        bb[b] = unite(bb[b], pb[p]);
-        bc[b]++;
-      }
+      if (!use[axis]) continue;
       double right_area[kBins];
       size_t right_n[kBins];
       DAabb acc = empty_box();
       size_t cnt = 0;
       for (int b = kBins - 1; b > 0; b--) {
-        acc = unite(acc, bb[b]);
-        cnt += bc[b];
+        acc = unite(acc, bb[axis][b]);
+        cnt += bc[axis][b];
         right_area[b] = half_area(acc);
         right_n[b] = cnt;
       }
       acc = empty_box();
       cnt = 0;
       for (int b = 1; b < kBins; b++) {
-        acc = unite(acc, bb[b - 1]);
-        cnt += bc[b - 1];
+        acc = unite(acc, bb[axis][b - 1]);
+        cnt += bc[axis][b - 1];
         if (cnt == 0 || right_n[b] == 0) continue;
         const double cost = kTraverse + kIntersect * (half_area(acc) * (double)cnt + right_area[b] * (double)right_n[b]) /
                                             std::max(parent, 1e-300);
@@ -238,20 +317,65 @@ struct SahBuilder {
     if (best_axis < 0) {  // every centre coincides: split the range in two
       mid = lo + n / 2;
     } else {
-      const double scale = kBins / (cb.mx[best_axis] - cb.mn[best_axis]);
-      auto first = out.indices.begin() + (std::ptrdiff_t)lo, last = out.indices.begin() + (std::ptrdiff_t)hi;
-      auto it = std::stable_partition(first, last, [&](int64_t p) {
-        int b = (int)((cen[3 * (size_t)p + best_axis] - cb.mn[best_axis]) * scale);
-        return std::min(std::max(b, 0), kBins - 1) < best_bin;
-      });
-      mid = (size_t)(it - out.indices.begin());
+      // stable partition by the split bin: the left records move down in order, the right
+      // ones go through the scratch buffer and follow them
+      const double sc = scale[best_axis], c0 = cb.mn[best_axis];
+      size_t w = lo, nr = 0;
+      right.resize(std::max(right.size(), n));
+      for (size_t i = lo; i < hi; i++) {
+        int b = (int)((rec[i].c[best_axis] - c0) * sc);
+        if (std::min(std::max(b, 0), kBins - 1) < best_bin) rec[w++] = rec[i];
+        else right[nr++] = rec[i];
+      }
+      std::copy(right.begin(), right.begin() + (std::ptrdiff_t)nr, rec.begin() + (std::ptrdiff_t)w);
+      mid = w;
       if (mid == lo || mid == hi) mid = lo + n / 2;
     }
-    const int32_t l = node(lo, mid, depth + 1);
-    const int32_t r = node(mid, hi, depth + 1);
-    out.nodes[(size_t)me].left = l;
-    out.nodes[(size_t)me].right = r;
-    out.nodes[(size_t)me].count = 0;
+    int32_t l, r;
+    if (depth <= kParallelDepth && n >= kParallelMin) {
+      // the right subtree on a thread of its own (into its own array, spliced after the left one)
+      std::vector<RefNode> rnodes;
+      std::vector<Prim> rright;
+      int rdepth = 0;
+      std::exception_ptr rerr;
+      std::thread th;
+      bool threaded = true;
+      try {
+        th = std::thread([&] {
+          try {
+            node(mid, hi, depth + 1, rnodes, rright, rdepth);
+          } catch (...) {
+            rerr = std::current_exception();
+          }
+        });
+      } catch (const std::system_error&) {
+        threaded = false;
+      }
+      if (!threaded) node(mid, hi, depth + 1, rnodes, rright, rdepth);
+      std::exception_ptr lerr;
+      try {
+        l = node(lo, mid, depth + 1, nodes, right, max_depth);
+      } catch (...) {
+        lerr = std::current_exception();
+      }
+      if (threaded) th.join();
+      if (lerr) std::rethrow_exception(lerr);
+      if (rerr) std::rethrow_exception(rerr);
+      const int32_t base = (int32_t)nodes.size();
+      for (RefNode& x : rnodes) {
+        if (x.left >= 0) x.left += base;
+        if (x.right >= 0) x.right += base;
+      }
+      nodes.insert(nodes.end(), rnodes.begin(), rnodes.end());
+      r = base;
+      max_depth = std::max(max_depth, rdepth);
+    } else {
+      l = node(lo, mid, depth + 1, nodes, right, max_depth);
+      r = node(mid, hi, depth + 1, nodes, right, max_depth);
+    }
+    nodes[(size_t)me].left = l;
+    nodes[(size_t)me].right = r;
+    nodes[(size_t)me].count = 0;
     return me;
   }
 };
@@ -289,9 +413,10 @@ TriF32 tri_f32(const double* v) {
 
 RefBvh build_sah_bvh(const std::vector<DAabb>& prim_bounds, size_t max_leaf) {
   SahBuilder b(prim_bounds, max_leaf);
+  std::vector<SahBuilder::Prim> scratch;
+  if (!prim_bounds.empty()) b.node(0, prim_bounds.size(), 1, b.out.nodes, scratch, b.out.depth);
   b.out.indices.resize(prim_bounds.size());
-  std::iota(b.out.indices.begin(), b.out.indices.end(), 0);
-  if (!prim_bounds.empty()) b.node(0, prim_bounds.size(), 1);
+  for (size_t i = 0; i < b.rec.size(); i++) b.out.indices[i] = b.rec[i].idx;
   return std::move(b.out);
 }
 
@@ -608,11 +733,36 @@ bool HostScene::build(const tpt_scene_desc* d) {
   tris32.resize((size_t)ntri);
   tri_orig.resize((size_t)ntri);
   tri_mat32.resize((size_t)ntri);
+  // Both trees of every mesh are built on host threads: each build is deterministic and
+  // independent of the others, and the results are laid out in mesh order below, so the scene
+  // is the same whatever the threads do.  Largest meshes first.
+  std::vector<int> by_size((size_t)nmesh);
+  std::iota(by_size.begin(), by_size.end(), 0);
+  std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) {
+    return mesh_tri_begin[(size_t)a + 1] - mesh_tri_begin[(size_t)a] > mesh_tri_begin[(size_t)b + 1] - mesh_tri_begin[(size_t)b];
+  });
+  std::vector<std::vector<DAabb>> bounds_of((size_t)nmesh);
+  std::vector<RefBvh> sah_of((size_t)nmesh);
+  {
+    std::vector<std::function<void()>> tasks;
+    for (int m : by_size)
+      tasks.push_back([&, m] {
+        const int64_t b0 = mesh_tri_begin[(size_t)m], b1 = mesh_tri_begin[(size_t)m + 1];
+        std::vector<DAabb>& bounds = bounds_of[(size_t)m];
+        bounds.resize((size_t)(b1 - b0));
+        for (int64_t t = b0; t < b1; t++) bounds[(size_t)(t - b0)] = triangle_bounds(&tri_v[9 * (size_t)t]);
+      });
+    run_parallel(tasks);
+    tasks.clear();
+    for (int m : by_size) {
+      tasks.push_back([&, m] { blas[(size_t)m] = build_ref_bvh(bounds_of[(size_t)m], 4); });  // LEAF_SIZE (constants.rs:21)
+      tasks.push_back([&, m] { sah_of[(size_t)m] = build_sah_bvh(bounds_of[(size_t)m], 4); });
+    }
+    run_parallel(tasks);
+  }
   for (int m = 0; m < nmesh; m++) {
     int64_t b0 = mesh_tri_begin[(size_t)m], b1 = mesh_tri_begin[(size_t)m + 1];
-    std::vector<DAabb> bounds((size_t)(b1 - b0));
-    for (int64_t t = b0; t < b1; t++) bounds[(size_t)(t - b0)] = triangle_bounds(&tri_v[9 * (size_t)t]);
-    blas[(size_t)m] = build_ref_bvh(bounds, 4);  // LEAF_SIZE (constants.rs:21)
+    std::vector<DAabb>().swap(bounds_of[(size_t)m]);
     const RefBvh& r = blas[(size_t)m];
     max_blas_depth = std::max(max_blas_depth, r.depth);
     MeshDev md{};
@@ -630,7 +780,7 @@ bool HostScene::build(const tpt_scene_desc* d) {
     for (int64_t ix : r.indices) ref_indices.push_back((int32_t)ix);
     // the FP32 tree (own SAH build) and its reordered triangles: slot b0 + k holds triangle
     // indices[k] of that tree
-    const RefBvh f = build_sah_bvh(bounds, 4);
+    const RefBvh f = std::move(sah_of[(size_t)m]);
     for (size_t k = 0; k < f.indices.size(); k++) {
       int64_t t = b0 + f.indices[k];
       const double* v = &tri_v[9 * (size_t)t];
