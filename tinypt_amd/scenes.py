@@ -157,8 +157,12 @@ def icosphere(subdivisions: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
     for _ in range(subdivisions):
         e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
         e.sort(axis=1)
-        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        # unique edges in row order, through one integer key per edge (lo * V + hi sorts as the
+        # rows do)
+        nv = np.int64(len(verts))
+        keys, inv = np.unique(e[:, 0] * nv + e[:, 1], return_inverse=True)
         inv = inv.reshape(-1)
+        uniq = np.stack([keys // nv, keys % nv], 1)
         mid = verts[uniq[:, 0]] + verts[uniq[:, 1]]
         mid /= np.linalg.norm(mid, axis=1, keepdims=True)
         base = len(verts)
